@@ -1,0 +1,130 @@
+"""Host-side mirror of the reference's operator interface for the step.
+
+The reference exposes (kernel.cu, Gas_Dinamic.cu):
+  * per-face operators  HLL / HLLC_2d_Korolkov / Godunov_Solver_Alexashov
+    (Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad) -> dt      kernel.cu:1399, 128;
+                                                     Gas_Dinamic.cu:1814
+  * the step kernel     Kernel_TVD(s, u, s2, u2, T, T_do, TT, method)
+                                                     kernel.cu:2513
+  * the time advance    funk_time(T, T_do, TT, i)    kernel.cu:1525
+This module offers the same names and argument meaning over torch device
+tensors, each call going through the C-ABI of libasteriy.so (HIP kernels for
+gfx950).  Tensors: s, u are (M, N, 2) float64 (FP64 path) or float32 (FP32
+path) on the GPU, the reference's double2 planes; T, T_do, TT are 1-element
+float64 device tensors; i is a 1-element int32 device tensor.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import native
+from .grid import EXACT, HLL as M_HLL, HLLC as M_HLLC
+
+_STATS_WORDS = 10
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _check_state(t, dtype, cells, name):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError(f"{name} must be a CUDA (HIP) tensor")
+    if t.dtype != dtype or not t.is_contiguous() or t.numel() != 2 * cells:
+        raise ValueError(f"{name}: need a contiguous {dtype} tensor of {cells} pairs, got "
+                         f"{t.dtype} {tuple(t.shape)}")
+
+
+def new_stats(device="cuda"):
+    """Zeroed ast_stats_t in device memory (10 x uint64 held as int64)."""
+    import torch
+    return torch.zeros(_STATS_WORDS, dtype=torch.int64, device=device)
+
+
+def Kernel_TVD(grid, s, u, s2, u2, T, T_do, TT, method: int, stats=None, mlp=None,
+               stream=None):
+    """One step of the reference's Kernel_TVD (kernel.cu:2513) on the GPU.
+
+    FP64 tensors run the bit-faithful cell-by-cell kernel (ast_step_f64);
+    FP32 tensors run the face-shared fast path (ast_step_f32), optionally
+    seeding the exact solver with a learned guess (`mlp`, see mlp.py)."""
+    import torch
+    cells = grid.N * grid.M
+    dt = s.dtype
+    if dt not in (torch.float64, torch.float32):
+        raise TypeError("state must be float64 or float32")
+    for t, nm in ((s, "s"), (u, "u"), (s2, "s2"), (u2, "u2")):
+        _check_state(t, dt, cells, nm)
+    gref = native.grid_ref(grid)
+    if dt == torch.float64:
+        if mlp is not None:
+            raise ValueError("the learned guess is served by the FP32 step only")
+        native.call("ast_step_f64", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
+                    _ptr(T_do), _ptr(TT), int(method), _ptr(stats), _stream(stream))
+    else:
+        native.call("ast_step_f32", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
+                    _ptr(T_do), _ptr(TT), int(method),
+                    _ptr(mlp.buffer) if mlp is not None else None, _ptr(stats),
+                    _stream(stream))
+
+
+def funk_time(T, T_do, TT, i=None, stream=None):
+    """kernel.cu:1525-1536: T_do <- T; TT += T_do; T <- 1e7; i += 1."""
+    native.call("ast_funk_time", _ptr(T), _ptr(T_do), _ptr(TT), _ptr(i), _stream(stream))
+
+
+def flux_batch(method: int, faces, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0, krit=0.05,
+               out=None, stats=None, stream=None):
+    """The per-face operators over n faces: faces is an (8, n) device tensor
+    (rhoL, pL, vxL, vyL, rhoR, pR, vxR, vyR); returns (5, n): mass flux,
+    energy flux, x- and y-momentum flux, dt."""
+    import torch
+    if faces.dim() != 2 or faces.shape[0] != 8 or not faces.is_contiguous():
+        raise ValueError("faces must be a contiguous (8, n) tensor")
+    n = faces.shape[1]
+    if out is None:
+        out = torch.empty((5, n), dtype=faces.dtype, device=faces.device)
+    if faces.dtype == torch.float64:
+        native.call("ast_flux_f64", int(method), n, _ptr(faces), float(n1), float(n2),
+                    float(rad), float(gamma), float(krit), _ptr(out), _ptr(stats),
+                    _stream(stream))
+    elif faces.dtype == torch.float32:
+        native.call("ast_flux_f32", int(method), n, _ptr(faces), float(n1), float(n2),
+                    float(rad), float(gamma), float(krit), _ptr(out), _ptr(stats),
+                    _stream(stream))
+    else:
+        raise TypeError("faces must be float64 or float32")
+    return out
+
+
+def HLL(faces, n1, n2, rad, **kw):
+    """kernel.cu:1399 over a batch of faces."""
+    return flux_batch(M_HLL, faces, n1, n2, rad, **kw)
+
+
+def HLLC_2d_Korolkov(faces, n1, n2, rad, **kw):
+    """Gas_Dinamic.cu:1814 (nul_potok=false) over a batch of faces."""
+    return flux_batch(M_HLLC, faces, n1, n2, rad, **kw)
+
+
+def Godunov_Solver_Alexashov(faces, n1, n2, rad, **kw):
+    """kernel.cu:128 over a batch of faces."""
+    return flux_batch(EXACT, faces, n1, n2, rad, **kw)
+
+
+def conservation_sums(grid, s, u, stream=None):
+    """Compensated FP64 (mass, energy) integrals of the state (device)."""
+    import torch
+    work = torch.empty(int(native.load().ast_conservation_work_doubles()), dtype=torch.float64,
+                       device=s.device)
+    native.call("ast_conservation_sums", native.grid_ref(grid), _ptr(s), _ptr(u),
+                1 if s.dtype == torch.float32 else 0, _ptr(work), _stream(stream))
+    return work[:2]

@@ -1,0 +1,952 @@
+/*
+ * asteriy_oracle.c -- CPU ORACLE (test infrastructure only).
+ *
+ * A line-faithful C restatement of the reference's explicit (z,r) step:
+ * the same formulas, constants, branch structure and floating-point
+ * operation order as the cited reference lines, compiled without FMA
+ * contraction (-ffp-contract=off).  It is the checker for the HIP kernels
+ * and the timed "reference CPU path"; only tests/, the smoke check and the
+ * bench's CPU-baseline leg load it.  Nothing in the product links it.
+ *
+ * Deliberate departures from the reference, each one documented:
+ *  - dt: the true minimum over all faces instead of the racy unsynchronised
+ *    store (kernel.cu:3013-3017);
+ *  - newton/devtwo stop after AST_ITER_CAP iterations and count it, where the
+ *    reference loops forever (kernel.cu:647-675, 710-740);
+ *  - the exact solver returns NaN fluxes when no wave pattern matches
+ *    (NaN input), where the reference reads uninitialised variables;
+ *  - boundary ghosts, the high-order face window and the frozen region are
+ *    runtime parameters (ast_grid_t) instead of the hard-wired choices of
+ *    Kernel_TVD/add_MK; the body-force band Q2 (kernel.cu:3034-3056) is not
+ *    restated (problem-specific, zero outside x in (20,30)).
+ *
+ * Parity status per function (how each is pinned, see tests/):
+ *  o_minmod/o_linear/o_linear2 : closed forms, special cases.
+ *  ora_hll                      : consistency, upwinding, exact mirror
+ *                                 antisymmetry, HLL closed form.
+ *  ora_hllc                     : consistency, upwinding, stationary contact,
+ *                                 mirror antisymmetry to round-off.
+ *  ora_godunov (+newton/devtwo) : independent Toro exact solver (toro.c),
+ *                                 Toro's published test table, the
+ *                                 reference's own `test` kernel KAT
+ *                                 (kernel.cu:3095-3107), root residuals.
+ *  ora_step                     : free-stream and rest-state preservation,
+ *                                 planar mass conservation, Sod convergence
+ *                                 to the exact solution, brute-force dt.
+ *  Branch-edge quirks (<= vs < region edges, eps-switched forms, the ptz
+ *  average) follow the reference text; no golden vector from the
+ *  reference exists to pin them (the reference ships no tests).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/asteriy.h"
+#include "asteriy_oracle.h"
+
+/* Header.h:62-63: the literals 10e-10 and 10e-8 are 1e-9 and 1e-7. */
+#define O_EPS 10e-10
+#define O_EPS8 10e-8
+
+typedef struct { double x, y; } d2;
+
+/* gamma-derived constants, Header.h:17-26 (g1, g2, gp, gm). */
+void ora_phys_init(ora_phys* ph, double gamma, double krit) {
+  ph->ga = gamma;
+  ph->g1 = gamma - 1.0;
+  ph->g2 = gamma + 1.0;
+  ph->gp = (ph->g2 / gamma) / 2.0;
+  ph->gm = (ph->g1 / gamma) / 2.0;
+  ph->krit = krit;
+}
+
+/* ---- L1 scalar primitives -------------------------------------------- */
+
+/* Gas_Dinamic.cu:858-872 */
+static int o_sign(double x) {
+  if (x > 0) return 1;
+  else if (x < 0) return -1;
+  else return 0;
+}
+
+/* kernel.cu:22-33 (CUDA min(double,double) is fmin) */
+static double o_minmod(double x, double y) {
+  if (o_sign(x) + o_sign(y) == 0) return 0.0;
+  return ((o_sign(x) + o_sign(y)) / 2.0) * fmin(fabs(x), fabs(y));
+}
+
+/* kernel.cu:35-39 */
+static double o_linear(double x1, double t1, double x2, double t2, double x3,
+                       double t3, double y) {
+  double d = o_minmod((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+  return (d * (y - x2) + t2);
+}
+
+/* kernel.cu:41-50 */
+static void o_linear2(double x1, double t1, double x2, double t2, double x3,
+                      double t3, double y1, double y2, double* A, double* B) {
+  double d = o_minmod((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+  *A = (d * (y1 - x2) + t2);
+  *B = (d * (y2 - x2) + t2);
+}
+
+/* kernel.cu:53-75 */
+static double o_my_min(double a, double b) { return (a <= b) ? a : b; }
+static double o_my_max(double a, double b) { return (a >= b) ? a : b; }
+
+/* exported for the primitive tests */
+double ora_minmod(double x, double y) { return o_minmod(x, y); }
+double ora_linear(double x1, double t1, double x2, double t2, double x3,
+                  double t3, double y) {
+  return o_linear(x1, t1, x2, t2, x3, t3, y);
+}
+void ora_linear2(double x1, double t1, double x2, double t2, double x3,
+                 double t3, double y1, double y2, double* A, double* B) {
+  o_linear2(x1, t1, x2, t2, x3, t3, y1, y2, A, B);
+}
+
+/* ---- L2 face-flux operators ------------------------------------------ */
+
+/* HLL, kernel.cu:1399-1523.  ggg == ga (Header.h:17-18); Omega = 0
+ * (Header.h:7). */
+double ora_hll(const ora_phys* ph, const double* Ls, const double* Lu,
+               const double* Rs, const double* Ru, double n1, double n2,
+               double* Ps, double* Pu, double rad) {
+  const double ggg = ph->ga;
+  const double Omega = 0.0;
+  double ro_L = Ls[0], p_L = Ls[1], v1_L = Lu[0], v2_L = Lu[1];
+  double ro_R = Rs[0], p_R = Rs[1], v1_R = Ru[0], v2_R = Ru[1];
+  double e_L, e_R, Vkv_L, Vkv_R, c_L, c_R;
+
+  Vkv_L = v1_L * v1_L + v2_L * v2_L;
+  Vkv_R = v1_R * v1_R + v2_R * v2_R;
+  if (ro_L <= 0) c_L = 0.0;
+  else c_L = sqrt(ggg * p_L / ro_L);
+  if (ro_R <= 0) c_R = 0.0;
+  else c_R = sqrt(ggg * p_R / ro_R);
+  e_L = p_L / (ggg - 1.0) + ro_L * Vkv_L / 2.0;
+  e_R = p_R / (ggg - 1.0) + ro_R * Vkv_R / 2.0;
+
+  double Vn_L = v1_L * n1 + v2_L * n2;
+  double Vn_R = v1_R * n1 + v2_R * n2;
+  double D_L = o_my_min(Vn_L, Vn_R) - o_my_max(c_L, c_R);
+  double D_R = o_my_max(Vn_L, Vn_R) + o_my_max(c_L, c_R);
+  double t = 10000000;
+  t = o_my_min(t, ph->krit * rad / o_my_max(fabs(D_L), fabs(D_R)));
+
+  double fx1 = ro_L * v1_L;
+  double fx2 = ro_L * v1_L * v1_L + p_L;
+  double fx3 = ro_L * v1_L * v2_L;
+  double fx5 = (e_L + p_L) * v1_L;
+  double fy1 = ro_L * v2_L;
+  double fy2 = ro_L * v1_L * v2_L;
+  double fy3 = ro_L * v2_L * v2_L + p_L;
+  double fy5 = (e_L + p_L) * v2_L;
+  double fl_1 = fx1 * n1 + fy1 * n2;
+  double fl_2 = fx2 * n1 + fy2 * n2;
+  double fl_3 = fx3 * n1 + fy3 * n2;
+  double fl_5 = fx5 * n1 + fy5 * n2;
+
+  if (D_L > Omega) {
+    Ps[0] = fl_1; Pu[0] = fl_2; Pu[1] = fl_3; Ps[1] = fl_5;
+    return t;
+  }
+  double hx1 = ro_R * v1_R;
+  double hx2 = ro_R * v1_R * v1_R + p_R;
+  double hx3 = ro_R * v1_R * v2_R;
+  double hx5 = (e_R + p_R) * v1_R;
+  double hy1 = ro_R * v2_R;
+  double hy2 = ro_R * v1_R * v2_R;
+  double hy3 = ro_R * v2_R * v2_R + p_R;
+  double hy5 = (e_R + p_R) * v2_R;
+  double fr_1 = hx1 * n1 + hy1 * n2;
+  double fr_2 = hx2 * n1 + hy2 * n2;
+  double fr_3 = hx3 * n1 + hy3 * n2;
+  double fr_5 = hx5 * n1 + hy5 * n2;
+  if (D_R < Omega) {
+    Ps[0] = fr_1; Pu[0] = fr_2; Pu[1] = fr_3; Ps[1] = fr_5;
+    return t;
+  }
+  double dq1 = ro_R - ro_L;
+  double dq2 = ro_R * v1_R - ro_L * v1_L;
+  double dq3 = ro_R * v2_R - ro_L * v2_L;
+  double dq5 = e_R - e_L;
+  Ps[0] = (D_R * fl_1 - D_L * fr_1 + D_L * D_R * dq1) / (D_R - D_L);
+  Pu[0] = (D_R * fl_2 - D_L * fr_2 + D_L * D_R * dq2) / (D_R - D_L);
+  Pu[1] = (D_R * fl_3 - D_L * fr_3 + D_L * D_R * dq3) / (D_R - D_L);
+  Ps[1] = (D_R * fl_5 - D_L * fr_5 + D_L * D_R * dq5) / (D_R - D_L);
+  return t;
+}
+
+/* HLLC_2d_Korolkov (Gas_Dinamic.cu:1814-1826) -> HLLC_2d_Korolkov_b_s
+ * (Gas_Dinamic.cu:1604-1811) with Q_L = Q_R = 1, W = 0, metod = 2,
+ * nul_potok = false.  The passive scalar (index 4), UZ0..UZ4 and sqrtro*
+ * have no effect on the returned values under these arguments and are not
+ * restated. */
+double ora_hllc(const ora_phys* ph, const double* Ls, const double* Lu,
+                const double* Rs, const double* Ru, double n1, double n2,
+                double* Ps, double* Pu, double rad) {
+  const double ggg = ph->ga, g1 = ph->g1;
+  const double W = 0.0;
+  double ro_L = Ls[0], p_L = Ls[1], v1_L = Lu[0], v2_L = Lu[1];
+  double ro_R = Rs[0], p_R = Rs[1], v1_R = Ru[0], v2_R = Ru[1];
+  double P[4];
+
+  double t1 = n2;
+  double t2 = -n1;
+  double u1 = v1_L * n1 + v2_L * n2;
+  double v1 = v1_L * t1 + v2_L * t2;
+  double u2 = v1_R * n1 + v2_R * n2;
+  double v2 = v1_R * t1 + v2_R * t2;
+
+  double cL = sqrt(ggg * p_L / ro_L);
+  double cR = sqrt(ggg * p_R / ro_R);
+  double uu_L = ((v1_L * v1_L) + (v2_L * v2_L)) / 2.0;
+  double uu_R = ((v1_R * v1_R) + (v2_R * v2_R)) / 2.0;
+
+  double SL = fmin(u1, u2) - fmax(cL, cR);
+  double SR = fmax(u1, u2) + fmax(cL, cR);
+  double suR = (SR - u2);
+  double suL = (SL - u1);
+  double SM = (suR * ro_R * u2 - suL * ro_L * u1 - p_R + p_L) /
+              (suR * ro_R - suL * ro_L);
+
+  double UU = fmax(fabs(SL), fabs(SR));
+  double time = ph->krit * rad / UU;
+
+  double FL[4], FR[4], UL[4], UR[4];
+  double e1 = p_L / g1 + ro_L * uu_L;
+  double e2 = p_R / g1 + ro_R * uu_R;
+
+  FL[0] = ro_L * u1;
+  FL[1] = ro_L * u1 * u1 + p_L;
+  FL[2] = ro_L * u1 * v1;
+  FL[3] = (e1 + p_L) * u1;
+  FR[0] = ro_R * u2;
+  FR[1] = ro_R * u2 * u2 + p_R;
+  FR[2] = ro_R * u2 * v2;
+  FR[3] = (e2 + p_R) * u2;
+  UL[0] = ro_L;
+  UL[1] = ro_L * u1;
+  UL[2] = ro_L * v1;
+  UL[3] = e1;
+  UR[0] = ro_R;
+  UR[1] = ro_R * u2;
+  UR[2] = ro_R * v2;
+  UR[3] = e2;
+
+  if (SL >= W) {
+    P[1] = n1 * (FL[1] - W * UL[1]) + t1 * (FL[2] - W * UL[2]);
+    P[2] = n2 * (FL[1] - W * UL[1]) + t2 * (FL[2] - W * UL[2]);
+    P[0] = FL[0] - W * UL[0];
+    P[3] = FL[3] - W * UL[3];
+    goto out;
+  }
+  if (SR <= W) {
+    P[1] = n1 * (FR[1] - W * UR[1]) + t1 * (FR[2] - W * UR[2]);
+    P[2] = n2 * (FR[1] - W * UR[1]) + t2 * (FR[2] - W * UR[2]);
+    P[0] = FR[0] - W * UR[0];
+    P[3] = FR[3] - W * UR[3];
+    goto out;
+  }
+  {
+    double ro_LL = ro_L * (SL - u1) / (SL - SM);
+    double ro_RR = ro_R * (SR - u2) / (SR - SM);
+    double suRm = suR / (SR - SM);
+    double suLm = suL / (SL - SM);
+    double ptzR = p_R + ro_R * suR * (SM - u2);
+    double ptzL = p_L + ro_L * suL * (SM - u1);
+    double ptz = (ptzR + ptzL) / 2.0;
+    double vLL = v1, vRR = v2;
+    double ee2 = e2 * suRm + (ptz * SM - p_R * u2) / (SR - SM);
+    double ee1 = e1 * suLm + (ptz * SM - p_L * u1) / (SL - SM);
+    double ULL[4], URR[4], PO[4];
+    ULL[0] = ro_LL; ULL[1] = ro_LL * SM; ULL[2] = ro_LL * vLL; ULL[3] = ee1;
+    URR[0] = ro_RR; URR[1] = ro_RR * SM; URR[2] = ro_RR * vRR; URR[3] = ee2;
+    if (SL < W && SM >= W) {
+      for (int i = 0; i < 4; i++)
+        PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
+    } else if (SR > W && SM < W) {
+      for (int i = 0; i < 4; i++)
+        PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
+    } else {
+      /* SM is NaN: the reference leaves PO uninitialised. */
+      for (int i = 0; i < 4; i++) PO[i] = NAN;
+    }
+    P[1] = n1 * PO[1] + t1 * PO[2];
+    P[2] = n2 * PO[1] + t2 * PO[2];
+    P[0] = PO[0];
+    P[3] = PO[3];
+  }
+out:
+  /* output map of HLLC_2d_Korolkov, Gas_Dinamic.cu:1821-1824 */
+  Ps[0] = P[0];
+  Ps[1] = P[3];
+  Pu[0] = P[1];
+  Pu[1] = P[2];
+  return time;
+}
+
+/* lev, kernel.cu:751-765 */
+static double o_lev(const ora_phys* ph, double enI, double pI, double rI,
+                    double enII, double pII, double rII, double uuu) {
+  double cI = sqrt(ph->ga * pI / rI);
+  double cII = sqrt(ph->ga * pII / rII);
+  double fI = (uuu - pI) / (rI * cI * sqrt(ph->gp * uuu / pI + ph->gm));
+  double fII = 2.0 / ph->g1 * cII * (pow((uuu / pII), ph->gm) - 1.0);
+  double f1 = fI + fII;
+  double f2 = enI - enII;
+  return f1 - f2;
+}
+
+/* newton, kernel.cu:635-680.  `guess` (if non-NULL) replaces the starting
+ * point pn; the stopping scale pee stays the PVRS-type estimate of
+ * kernel.cu:641-644 either way. */
+static double o_newton(const ora_phys* ph, double enI, double pI, double rI,
+                       double enII, double pII, double rII,
+                       const double* guess, int* iters, int* capped) {
+  const double ga = ph->ga, gp = ph->gp, gm = ph->gm;
+  double fI, fIs, fII, fIIs, p;
+  double cI = sqrt(ga * pI / rI);
+  double cII = sqrt(ga * pII / rII);
+  double pn = pI * rII * cII + pII * rI * cI + (enI - enII) * rI * cI * rII * cII;
+  pn = pn / (rI * cI + rII * cII);
+  double pee = pn;
+  if (guess) pn = *guess;
+  int kiter = 0;
+  for (;;) {
+    p = pn;
+    kiter = kiter + 1;
+    fI = (p - pI) / (rI * cI * sqrt(gp * p / pI + gm));
+    fIs = (ga + 1.0) * p / pI + (3.0 * ga - 1.0);
+    fIs = fIs / (4.0 * ga * rI * cI * pow((gp * p / pI + gm), (3.0 / 2.0)));
+    fII = (p - pII) / (rII * cII * sqrt(gp * p / pII + gm));
+    fIIs = (ga + 1.0) * p / pII + (3.0 * ga - 1.0);
+    fIIs = fIIs / (4.0 * ga * rII * cII * pow((gp * p / pII + gm), (3.0 / 2.0)));
+    pn = p - (fI + fII - (enI - enII)) / (fIs + fIIs);
+    if (!(fabs(pn / pee - p / pee) >= O_EPS)) break;
+    if (kiter >= AST_ITER_CAP) { *capped = 1; break; }
+  }
+  *iters = kiter;
+  return pn;
+}
+
+/* devtwo, kernel.cu:682-749 (bisection on [pI, pII]) */
+static double o_devtwo(const ora_phys* ph, double enI, double pI, double rI,
+                       double enII, double pII, double rII, int* iters,
+                       int* capped) {
+  const double epsil = 10e-10;
+  double kl = pI, kp = pII, kc, ksi, ksir, ksit;
+  int kpizd = 0;
+  ksi = o_lev(ph, enI, pI, rI, enII, pII, rII, kl);
+  ksir = o_lev(ph, enI, pI, rI, enII, pII, rII, kp);
+  *iters = 0;
+  if (fabs(ksi) <= epsil) return kl;
+  if (fabs(ksir) <= epsil) return kp;
+  for (;;) {
+    kpizd = kpizd + 1;
+    kc = (kl + kp) / 2.0;
+    ksit = o_lev(ph, enI, pI, rI, enII, pII, rII, kc);
+    if (fabs(ksit) <= epsil) break;
+    if ((ksi * ksit) <= 0.0) {
+      kp = kc;
+      ksir = ksit;
+    } else {
+      kl = kc;
+      ksi = ksit;
+    }
+    if (kpizd >= AST_ITER_CAP) { *capped = 1; break; }
+  }
+  (void)ksir;
+  *iters = kpizd;
+  return kc;
+}
+
+/* Godunov_Solver_Alexashov, kernel.cu:128-570 (exact Riemann solution at the
+ * face, w = 0).  `guess`: optional Newton starting point (learned guess).
+ * `info` (may be NULL) receives the solve's internals. */
+double ora_godunov_ex(const ora_phys* ph, const double* Ls, const double* Lu,
+                      const double* Rs, const double* Ru, double n1, double n2,
+                      double* Ps, double* Pu, double rad, const double* guess,
+                      ora_info* info) {
+  const double ga = ph->ga, g1 = ph->g1, g2 = ph->g2, gm = ph->gm;
+  const double gg1 = g1, gg2 = g2, gga = ga;
+  const double krit = ph->krit;
+  double w = 0.0;
+  double al = n1, be = n2, ge = 0.0;
+  double time = 0.0;
+  double al2 = -n2, be2 = n1, ge2 = 0.0;
+  double al3 = 0.0, be3 = 0.0, ge3 = 1.0;
+
+  double enI = al * Lu[0] + be * Lu[1];
+  double teI2 = al2 * Lu[0] + be2 * Lu[1];
+  double teI3 = al3 * Lu[0] + be3 * Lu[1];
+  double enII = al * Ru[0] + be * Ru[1];
+  double teII2 = al2 * Ru[0] + be2 * Ru[1];
+  double teII3 = al3 * Ru[0] + be3 * Ru[1];
+  double pI = Ls[1], pII = Rs[1], rI = Ls[0], rII = Rs[0];
+
+  int ipiz = 0;
+  if (pI > pII) {
+    double eno2 = enII, teo22 = teII2, teo23 = teII3, p2 = pII, r2 = rII;
+    double eno1 = enI, teo12 = teI2, teo13 = teI3, p1 = pI, r1 = rI;
+    enI = -eno2; teI2 = teo22; teI3 = teo23; pI = p2; rI = r2;
+    enII = -eno1; teII2 = teo12; teII3 = teo13; pII = p1; rII = r1;
+    w = -w;
+    ipiz = 1;
+  }
+
+  double cI = 0.0, cII = 0.0;
+  if (rI != 0.0) cI = sqrt(ga * pI / rI);
+  if (rII != 0.0) cII = sqrt(ga * pII / rII);
+
+  double a = sqrt(rI * (g2 * pII + g1 * pI) / 2.0);
+  double Uud = (pII - pI) / a;
+  double Urz = -2.0 * cII / g1 * (1.0 - pow((pI / pII), gm));
+  double Uvk = -2.0 * (cII + cI) / g1;
+  double Udf = enI - enII;
+
+  int il = 9, ip = 9, iters = 0, capped = 0;
+  double p = NAN, r = NAN, te2 = NAN, te3 = NAN, en = NAN;
+
+  if (Udf < Uvk) {
+    il = -1; ip = -1;
+  } else if ((Udf >= Uvk) && (Udf <= Urz)) {
+    p = pI * pow(((Udf - Uvk) / (Urz - Uvk)), (1.0 / gm));
+    il = 0; ip = 0;
+  } else if ((Udf > Urz) && (Udf <= Uud)) {
+    p = o_devtwo(ph, enI, pI, rI, enII, pII, rII, &iters, &capped);
+    il = 1; ip = 0;
+  } else if (Udf > Uud) {
+    p = o_newton(ph, enI, pI, rI, enII, pII, rII, guess, &iters, &capped);
+    il = 1; ip = 1;
+  }
+  double pstar = p, ustar = NAN;
+
+  /* TWO SHOCKS, kernel.cu:234-288 */
+  if ((il == 1) && (ip == 1)) {
+    double aI = sqrt(rI * (g2 / 2.0 * p + g1 / 2.0 * pI));
+    double aII = sqrt(rII * (g2 / 2.0 * p + g1 / 2.0 * pII));
+    double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+    double dI = enI - aI / rI;
+    double dII = enII + aII / rII;
+    ustar = u;
+    double UU = fmax(fabs(dI), fabs(dII));
+    if (UU > O_EPS8) time = krit * rad / UU;
+    else time = krit * rad / O_EPS8;
+    if (w <= dI) {
+      en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3;
+    } else if ((w > dI) && (w <= u)) {
+      en = u; r = rI * aI / (aI - rI * (enI - u)); te2 = teI2; te3 = teI3;
+    } else if ((w > u) && (w < dII)) {
+      en = u; r = rII * aII / (aII + rII * (enII - u)); te2 = teII2; te3 = teII3;
+    } else if (w >= dII) {
+      en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3;
+    }
+  }
+
+  /* LEFT SHOCK, RIGHT EXPANSION FAN, kernel.cu:292-364 */
+  if ((il == 1) && (ip == 0)) {
+    double aI = sqrt(rI * (g2 / 2.0 * p + g1 / 2.0 * pI));
+    double aII;
+    if (fabs(p - pII) < O_EPS) aII = rII * cII;
+    else aII = gm * rII * cII * (1.0 - p / pII) / (1.0 - pow((p / pII), gm));
+    double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+    double dI = enI - aI / rI;
+    double dII = enII + cII;
+    double ddII = u + cII - g1 * (enII - u) / 2.0;
+    ustar = u;
+    double UU = fmax(fabs(dI), fabs(dII));
+    UU = fmax(UU, fabs(ddII));
+    if (UU > O_EPS8) time = krit * rad / UU;
+    else time = krit * rad / O_EPS8;
+    if (w <= dI) {
+      en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3;
+    }
+    if ((w > dI) && (w <= u)) {
+      en = u; r = rI * aI / (aI - rI * (enI - u)); te2 = teI2; te3 = teI3;
+    }
+    if ((w > u) && (w <= ddII)) {
+      double ce = cII - g1 / 2.0 * (enII - u);
+      en = u; r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+    }
+    if ((w > ddII) && (w < dII)) {
+      double ce = -g1 / g2 * (enII - w) + 2.0 / g2 * cII;
+      en = w - ce;
+      p = pII * pow((ce / cII), (1.0 / gm));
+      r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+    }
+    if (w >= dII) {
+      en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3;
+    }
+  }
+
+  /* TWO EXPANSION FANS, kernel.cu:366-477 */
+  if ((il == 0) && (ip == 0)) {
+    double aI, aII;
+    if (fabs(p - pI) < O_EPS) aI = rI * cI;
+    else aI = gm * rI * cI * (1.0 - p / pI) / (1.0 - pow((p / pI), gm));
+    if (fabs(p - pII) < O_EPS) aII = rII * cII;
+    else aII = gm * rII * cII * (1.0 - p / pII) / (1.0 - pow((p / pII), gm));
+    double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+    double dI = enI - cI;
+    double ddI = u - cI - g1 * (enI - u) / 2.0;
+    double dII = enII + cII;
+    double ddII = u + cII - g1 * (enII - u) / 2.0;
+    ustar = u;
+    double UU = fmax(fabs(dI), fabs(dII));
+    UU = fmax(UU, fabs(ddII));
+    UU = fmax(UU, fabs(ddI));
+    if (UU > O_EPS8) time = krit * rad / UU;
+    else time = krit * rad / O_EPS8;
+    if (w <= dI) {
+      en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3;
+    }
+    if ((w > dI) && (w < ddI)) {
+      double ce = g1 / g2 * (enI - w) + 2.0 / g2 * cI;
+      en = w + ce;
+      p = pI * pow((ce / cI), (1.0 / gm));
+      r = ga * p / ce / ce; te2 = teI2; te3 = teI3;
+    }
+    if ((w >= ddI) && (w <= u)) {
+      double ce = cI + g1 / 2.0 * (enI - u);
+      en = u; r = ga * p / ce / ce; te2 = teI2; te3 = teI3;
+    }
+    if ((w > u) && (w <= ddII)) {
+      double ce = cII - g1 / 2.0 * (enII - u);
+      en = u; r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+    }
+    if ((w > ddII) && (w < dII)) {
+      double ce = -g1 / g2 * (enII - w) + 2.0 / g2 * cII;
+      en = w - ce;
+      p = pII * pow((ce / cII), (1.0 / gm));
+      r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+    }
+    if (w >= dII) {
+      en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3;
+    }
+  }
+
+  /* VACUUM, kernel.cu:480-544 */
+  if ((il == -1) && (ip == -1)) {
+    double dI = enI - cI;
+    double ddI = enI + 2.0 / gg1 * cI;
+    double dII = enII + cII;
+    double ddII = enII - 2.0 / gg1 * cII;
+    double UU = fmax(fabs(dI), fabs(dII));
+    UU = fmax(UU, fabs(ddII));
+    UU = fmax(UU, fabs(ddI));
+    if (UU > O_EPS8) time = krit * rad / UU;
+    else time = krit * rad / O_EPS8;
+    if (w <= dI) {
+      en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3;
+    }
+    if ((w > dI) && (w < ddI)) {
+      double ce = gg1 / gg2 * (enI - w) + 2.0 / gg2 * cI;
+      en = w + ce;
+      p = pI * pow((ce / cI), (1.0 / gm));
+      r = gga * p / ce / ce; te2 = teI2; te3 = teI3;
+    }
+    if ((w >= ddI) && (w <= ddII)) {
+      en = w; p = 0.0; r = 0.0; te2 = 0.0; te3 = 0.0;
+    }
+    if ((w > ddII) && (w < dII)) {
+      double ce = -gg1 / gg2 * (enII - w) + 2.0 / gg2 * cII;
+      en = w - ce;
+      p = pII * pow((ce / cII), (1.0 / gm));
+      r = gga * p / ce / ce; te2 = teII2; te3 = teII3;
+    }
+    if (w >= dII) {
+      en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3;
+    }
+  }
+
+  if (ipiz == 1) {
+    en = -en;
+    w = -w;
+  }
+
+  double uo = al * en + al2 * te2 + al3 * te3;
+  double vo = be * en + be2 * te2 + be3 * te3;
+  double wo = ge * en + ge2 * te2 + ge3 * te3;
+  double eo = p / g1 + 0.5 * r * (uo * uo + vo * vo + wo * wo);
+  en = al * uo + be * vo + ge * wo;
+
+  Ps[0] = (r * (en - w));
+  Pu[0] = (r * (en - w) * uo + al * p);
+  Pu[1] = (r * (en - w) * vo + be * p);
+  Ps[1] = ((en - w) * eo + en * p);
+
+  if (info) {
+    info->pattern = (il == 9) ? -9 : (il == -1 ? -1 : il + ip);
+    info->iters = iters;
+    info->capped = capped;
+    info->swapped = ipiz;
+    info->p_star = pstar;
+    info->u_star = ustar;
+    info->enI = enI; info->pI = pI; info->rI = rI;
+    info->enII = enII; info->pII = pII; info->rII = rII;
+  }
+  return time;
+}
+
+double ora_godunov(const ora_phys* ph, const double* Ls, const double* Lu,
+                   const double* Rs, const double* Ru, double n1, double n2,
+                   double* Ps, double* Pu, double rad) {
+  return ora_godunov_ex(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad, NULL, NULL);
+}
+
+double ora_flux(int method, const ora_phys* ph, const double* Ls,
+                const double* Lu, const double* Rs, const double* Ru,
+                double n1, double n2, double* Ps, double* Pu, double rad) {
+  switch (method) {
+    case AST_HLL: return ora_hll(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    case AST_HLLC: return ora_hllc(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    default: return ora_godunov(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+  }
+}
+
+/* Batched faces (SoA, the layout of ast_flux_f64).  iters (may be NULL)
+ * receives each face's newton/devtwo iteration count and pattern in
+ * iters[2*i], iters[2*i+1]. */
+void ora_flux_batch(int method, double gamma, double krit, long n,
+                    const double* in, double n1, double n2, double rad,
+                    double* out, int* iters, int nthreads) {
+  ora_phys ph;
+  ora_phys_init(&ph, gamma, krit);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (long i = 0; i < n; i++) {
+    double Ls[2] = {in[0 * n + i], in[1 * n + i]};
+    double Lu[2] = {in[2 * n + i], in[3 * n + i]};
+    double Rs[2] = {in[4 * n + i], in[5 * n + i]};
+    double Ru[2] = {in[6 * n + i], in[7 * n + i]};
+    double Ps[2], Pu[2], t;
+    if (method == AST_EXACT) {
+      ora_info inf;
+      t = ora_godunov_ex(&ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad, NULL, &inf);
+      if (iters) { iters[2 * i] = inf.iters; iters[2 * i + 1] = inf.pattern; }
+    } else {
+      t = ora_flux(method, &ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    }
+    out[0 * n + i] = Ps[0];
+    out[1 * n + i] = Ps[1];
+    out[2 * n + i] = Pu[0];
+    out[3 * n + i] = Pu[1];
+    out[4 * n + i] = t;
+  }
+}
+
+/* Exact solver with an explicit Newton starting point per face (NaN = use
+ * the PVRS guess); reports iterations.  Used by the learned-guess tests. */
+void ora_godunov_guess_batch(double gamma, double krit, long n, const double* in,
+                             double n1, double n2, double rad,
+                             const double* guess, double* out, int* iters,
+                             double* pstar) {
+  ora_phys ph;
+  ora_phys_init(&ph, gamma, krit);
+  for (long i = 0; i < n; i++) {
+    double Ls[2] = {in[0 * n + i], in[1 * n + i]};
+    double Lu[2] = {in[2 * n + i], in[3 * n + i]};
+    double Rs[2] = {in[4 * n + i], in[5 * n + i]};
+    double Ru[2] = {in[6 * n + i], in[7 * n + i]};
+    double Ps[2], Pu[2];
+    ora_info inf;
+    const double* gp = (guess && !isnan(guess[i])) ? &guess[i] : NULL;
+    double t = ora_godunov_ex(&ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad, gp, &inf);
+    if (out) {
+      out[0 * n + i] = Ps[0];
+      out[1 * n + i] = Ps[1];
+      out[2 * n + i] = Pu[0];
+      out[3 * n + i] = Pu[1];
+      out[4 * n + i] = t;
+    }
+    if (iters) { iters[2 * i] = inf.iters; iters[2 * i + 1] = inf.pattern; }
+    if (pstar) pstar[i] = inf.p_star;
+  }
+}
+
+/* ---- L3: the step -------------------------------------------------------- */
+
+/* Ghost state for an index outside [0,N) x [0,M): the rules of
+ * Kernel_TVD (kernel.cu:2549-2698) and add_MK (kernel.cu:2328-2377),
+ * expressed per side.  Only called with exactly one index out of range
+ * (the stencil is a cross). */
+static void o_fetch(const ast_grid_t* g, const d2* s, const d2* u, int n,
+                    int m, d2* so, d2* uo) {
+  const int N = g->N, M = g->M;
+  int side = -1, in_n = n, in_m = m;
+  if (n < 0) { side = AST_XLO; in_n = -1 - n; }
+  else if (n >= N) { side = AST_XHI; in_n = 2 * N - 1 - n; }
+  else if (m < 0) { side = AST_YLO; in_m = -1 - m; }
+  else if (m >= M) { side = AST_YHI; in_m = 2 * M - 1 - m; }
+  if (side < 0) {
+    *so = s[(long)m * N + n];
+    *uo = u[(long)m * N + n];
+    return;
+  }
+  int bc = g->bc[side];
+  if (bc == AST_BC_DIRICHLET) {
+    so->x = g->bc_state[side][0];
+    so->y = g->bc_state[side][1];
+    uo->x = g->bc_state[side][2];
+    uo->y = g->bc_state[side][3];
+    return;
+  }
+  if (bc == AST_BC_REFLECT) {
+    long k = (long)in_m * N + in_n;
+    *so = s[k];
+    *uo = u[k];
+    if (side <= AST_XHI) uo->x = -uo->x;
+    else uo->y = -uo->y;
+    return;
+  }
+  /* outflow: the edge cell */
+  int en = n < 0 ? 0 : (n >= N ? N - 1 : n);
+  int em = m < 0 ? 0 : (m >= M ? M - 1 : m);
+  long k = (long)em * N + en;
+  *so = s[k];
+  *uo = u[k];
+  if (bc == AST_BC_OUTFLOW_CLAMPED) {
+    double c = g->bc_vclamp[side];
+    /* kernel.cu:2347-2350: the lo-side normal velocity is capped from
+     * above (only outflow towards -x passes); the hi side mirrors it. */
+    if (side == AST_XLO) { if (uo->x > c) uo->x = c; }
+    else if (side == AST_XHI) { if (uo->x < c) uo->x = c; }
+    else if (side == AST_YLO) { if (uo->y > c) uo->y = c; }
+    else { if (uo->y < c) uo->y = c; }
+  }
+}
+
+static int o_frozen(const ast_grid_t* g, double x, double y) {
+  if (!(g->frozen_ax > 0.0)) return 0;
+  double dxr = x - g->frozen_x0;
+  double d2v = (dxr * dxr) / (g->frozen_ax * g->frozen_ax) +
+               (y * y) / (g->frozen_ay * g->frozen_ay);
+  return d2v < 1.0;
+}
+
+static int o_ho(const ast_grid_t* g, double xf) {
+  return xf > g->ho_lo && xf < g->ho_hi;
+}
+
+typedef double (*o_flux_fn)(const ora_phys*, const double*, const double*,
+                            const double*, const double*, double, double,
+                            double*, double*, double);
+
+/* One cell of Kernel_TVD (kernel.cu:2513-3073) with the update of
+ * kernel.cu:3059-3073 (planar) or EnergyConservationLaws
+ * (KorolkovSolver.cu:13-32, axisymmetric).  Returns the cell's tmin. */
+static double o_cell(const ast_grid_t* g, const ora_phys* ph, const d2* s,
+                     const d2* u, d2* s2, d2* u2, double T_do, int method,
+                     int n, int m) {
+  const int N = g->N;
+  const double dx = g->dx, dy = g->dy, ggg = ph->ga;
+  const long index = (long)m * N + n;
+  double y = g->y_min + m * dy;
+  double x = g->x_min + n * dx;
+  d2 s_1 = s[index], u_1 = u[index];
+  d2 s_2, s_3, s_4, s_5, u_2, u_3, u_4, u_5;
+  d2 s21, s31, s41, s51, u21, u31, u41, u51;
+  d2 s12, s13, s14, s15, u12, u13, u14, u15;
+  double Ps12[2] = {0, 0}, Pu12[2] = {0, 0}, Ps13[2] = {0, 0}, Pu13[2] = {0, 0};
+  double Ps14[2] = {0, 0}, Pu14[2] = {0, 0}, Ps15[2] = {0, 0}, Pu15[2] = {0, 0};
+  double tmin = 1000;
+  double A = 0, B = 0;
+  o_flux_fn F = method == AST_HLL ? ora_hll
+              : method == AST_HLLC ? ora_hllc : ora_godunov;
+
+  if (o_frozen(g, x, y)) {  /* kernel.cu:2317-2324: copied, no dt vote */
+    s2[index] = s_1;
+    u2[index] = u_1;
+    return INFINITY;
+  }
+
+  /* gather, kernel.cu:2549-2698 */
+  o_fetch(g, s, u, n + 1, m, &s_2, &u_2);
+  o_fetch(g, s, u, n, m + 1, &s_5, &u_5);
+  o_fetch(g, s, u, n - 1, m, &s_4, &u_4);
+  o_fetch(g, s, u, n, m - 1, &s_3, &u_3);
+  o_fetch(g, s, u, n + 2, m, &s21, &u21);
+  o_fetch(g, s, u, n - 2, m, &s41, &u41);
+  o_fetch(g, s, u, n, m + 2, &s51, &u51);
+  o_fetch(g, s, u, n, m - 2, &s31, &u31);
+
+  /* own-side face states, kernel.cu:2701-2797 (grad_p = true) */
+  o_linear2(x - dx, s_4.x, x, s_1.x, x + dx, s_2.x, x - dx / 2.0, x + dx / 2.0, &A, &B);
+  s12.x = (B <= 0) ? s_1.x : B;
+  s14.x = (A <= 0) ? s_1.x : A;
+  o_linear2(x - dx, s_4.y, x, s_1.y, x + dx, s_2.y, x - dx / 2.0, x + dx / 2.0, &A, &B);
+  s12.y = (B <= 0) ? s_1.y : B;
+  s14.y = (A <= 0) ? s_1.y : A;
+  o_linear2(x - dx, u_4.x, x, u_1.x, x + dx, u_2.x, x - dx / 2.0, x + dx / 2.0, &A, &B);
+  u12.x = B;
+  u14.x = A;
+  o_linear2(x - dx, u_4.y, x, u_1.y, x + dx, u_2.y, x - dx / 2.0, x + dx / 2.0, &A, &B);
+  u12.y = B;
+  u14.y = A;
+  o_linear2(y - dy, s_3.x, y, s_1.x, y + dy, s_5.x, y - dy / 2.0, y + dy / 2.0, &A, &B);
+  s15.x = (B <= 0) ? s_1.x : B;
+  s13.x = (A <= 0) ? s_1.x : A;
+  o_linear2(y - dy, s_3.y, y, s_1.y, y + dy, s_5.y, y - dy / 2.0, y + dy / 2.0, &A, &B);
+  s15.y = (B <= 0) ? s_1.y : B;
+  s13.y = (A <= 0) ? s_1.y : A;
+  o_linear2(y - dy, u_3.x, y, u_1.x, y + dy, u_5.x, y - dy / 2.0, y + dy / 2.0, &A, &B);
+  u15.x = B;
+  u13.x = A;
+  o_linear2(y - dy, u_3.y, y, u_1.y, y + dy, u_5.y, y - dy / 2.0, y + dy / 2.0, &A, &B);
+  u15.y = B;
+  u13.y = A;
+
+  /* east face, kernel.cu:2799-2813 */
+  if (o_ho(g, x + dx / 2.0)) {
+    s21.x = o_linear(x, s_1.x, x + dx, s_2.x, x + 2.0 * dx, s21.x, x + dx / 2.0);
+    if (s21.x <= 0) s21.x = s_2.x;
+    s21.y = o_linear(x, s_1.y, x + dx, s_2.y, x + 2.0 * dx, s21.y, x + dx / 2.0);
+    if (s21.y <= 0) s21.y = s_2.y;
+    u21.x = o_linear(x, u_1.x, x + dx, u_2.x, x + 2.0 * dx, u21.x, x + dx / 2.0);
+    u21.y = o_linear(x, u_1.y, x + dx, u_2.y, x + 2.0 * dx, u21.y, x + dx / 2.0);
+    tmin = fmin(tmin, F(ph, &s12.x, &u12.x, &s21.x, &u21.x, 1, 0, Ps12, Pu12, dx));
+  } else {
+    tmin = o_my_min(tmin, ora_hll(ph, &s_1.x, &u_1.x, &s_2.x, &u_2.x, 1, 0, Ps12, Pu12, dx));
+  }
+
+  /* south and north faces, kernel.cu:2816-2840 (tested at the cell x) */
+  if (o_ho(g, x)) {
+    s31.x = o_linear(y, s_1.x, y - dy, s_3.x, y - 2.0 * dy, s31.x, y - dy / 2.0);
+    if (s31.x <= 0) s31.x = s_3.x;
+    s31.y = o_linear(y, s_1.y, y - dy, s_3.y, y - 2.0 * dy, s31.y, y - dy / 2.0);
+    if (s31.y <= 0) s31.y = s_3.y;
+    u31.x = o_linear(y, u_1.x, y - dy, u_3.x, y - 2.0 * dy, u31.x, y - dy / 2.0);
+    u31.y = o_linear(y, u_1.y, y - dy, u_3.y, y - 2.0 * dy, u31.y, y - dy / 2.0);
+
+    s51.x = o_linear(y, s_1.x, y + dy, s_5.x, y + 2.0 * dy, s51.x, y + dy / 2.0);
+    if (s51.x <= 0) s51.x = s_5.x;
+    s51.y = o_linear(y, s_1.y, y + dy, s_5.y, y + 2.0 * dy, s51.y, y + dy / 2.0);
+    if (s51.y <= 0) s51.y = s_5.y;
+    u51.x = o_linear(y, u_1.x, y + dy, u_5.x, y + 2.0 * dy, u51.x, y + dy / 2.0);
+    u51.y = o_linear(y, u_1.y, y + dy, u_5.y, y + 2.0 * dy, u51.y, y + dy / 2.0);
+
+    tmin = fmin(tmin, F(ph, &s13.x, &u13.x, &s31.x, &u31.x, 0, -1, Ps13, Pu13, dy));
+    tmin = fmin(tmin, F(ph, &s15.x, &u15.x, &s51.x, &u51.x, 0, 1, Ps15, Pu15, dy));
+  } else {
+    tmin = o_my_min(tmin, ora_hll(ph, &s_1.x, &u_1.x, &s_3.x, &u_3.x, 0, -1, Ps13, Pu13, dy));
+    tmin = o_my_min(tmin, ora_hll(ph, &s_1.x, &u_1.x, &s_5.x, &u_5.x, 0, 1, Ps15, Pu15, dy));
+  }
+
+  /* west face, kernel.cu:2843-2857 */
+  if (o_ho(g, x - dx / 2.0)) {
+    s41.x = o_linear(x, s_1.x, x - dx, s_4.x, x - 2.0 * dx, s41.x, x - dx / 2.0);
+    if (s41.x <= 0) s41.x = s_4.x;
+    s41.y = o_linear(x, s_1.y, x - dx, s_4.y, x - 2.0 * dx, s41.y, x - dx / 2.0);
+    if (s41.y <= 0) s41.y = s_4.y;
+    u41.x = o_linear(x, u_1.x, x - dx, u_4.x, x - 2.0 * dx, u41.x, x - dx / 2.0);
+    u41.y = o_linear(x, u_1.y, x - dx, u_4.y, x - 2.0 * dx, u41.y, x - dx / 2.0);
+    tmin = fmin(tmin, F(ph, &s14.x, &u14.x, &s41.x, &u41.x, -1, 0, Ps14, Pu14, dx));
+  } else {
+    tmin = o_my_min(tmin, ora_hll(ph, &s_1.x, &u_1.x, &s_4.x, &u_4.x, -1, 0, Ps14, Pu14, dx));
+  }
+
+  /* flux assembly, kernel.cu:3027-3032 */
+  double PSx = Ps12[0] * dy + Ps13[0] * dx + Ps14[0] * dy + Ps15[0] * dx;
+  double PSy = Ps12[1] * dy + Ps13[1] * dx + Ps14[1] * dy + Ps15[1] * dx;
+  double PUx = Pu12[0] * dy + Pu13[0] * dx + Pu14[0] * dy + Pu15[0] * dx;
+  double PUy = Pu12[1] * dy + Pu13[1] * dx + Pu14[1] * dy + Pu15[1] * dx;
+  double dV = dx * dy;
+  d2 r, v;
+
+  if (g->geometry == AST_AXISYMMETRIC) {
+    /* EnergyConservationLaws, KorolkovSolver.cu:13-32 */
+    r.x = s_1.x - (T_do / dV) * PSx - T_do * s_1.x * u_1.y / y;
+    if (r.x <= 0) r.x = s_1.x;
+    v.x = (s_1.x * u_1.x - (T_do / dV) * PUx - T_do * s_1.x * u_1.y * u_1.x / y) / r.x;
+    v.y = (s_1.x * u_1.y - (T_do / dV) * PUy - T_do * s_1.x * u_1.y * u_1.y / y) / r.x;
+    r.y = (((s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) -
+            (T_do / dV) * PSy -
+            T_do * u_1.y * (ggg * s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) / y) -
+           0.5 * r.x * (v.x * v.x + v.y * v.y)) * (ggg - 1);
+    if (r.y <= 0) r.y = 0.000001;
+  } else {
+    /* kernel.cu:3059-3073 with Q2 = Q2x = Q3 = 0 */
+    r.x = s_1.x - (T_do / dV) * PSx;
+    if (r.x <= 0.0) r.x = s_1.x;
+    v.x = (s_1.x * u_1.x - (T_do / dV) * PUx) / r.x;
+    v.y = (s_1.x * u_1.y - (T_do / dV) * PUy) / r.x;
+    r.y = (((s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) -
+            (T_do / dV) * PSy) -
+           0.5 * r.x * (v.x * v.x + v.y * v.y)) * (ggg - 1);
+    if (r.y <= 0.0) r.y = 0.000001;
+  }
+  s2[index] = r;
+  u2[index] = v;
+  return tmin;
+}
+
+/* The step over the whole grid.  s,u,s2,u2 are N*M pairs (interleaved
+ * doubles, the reference's double2 planes).  *T is lowered to the minimum
+ * cell tmin (kernel.cu:3013-3017, made exact). */
+void ora_step(const ast_grid_t* g, const double* s, const double* u,
+              double* s2, double* u2, double* T, const double* T_do,
+              int method, int nthreads) {
+  ora_phys ph;
+  ora_phys_init(&ph, g->gamma, g->krit);
+  double tmin_all = *T;
+  const double tdo = *T_do;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(min : tmin_all)
+#endif
+  for (int m = 0; m < g->M; m++) {
+    for (int n = 0; n < g->N; n++) {
+      double t = o_cell(g, &ph, (const d2*)s, (const d2*)u, (d2*)s2, (d2*)u2,
+                        tdo, method, n, m);
+      if (tmin_all > t) tmin_all = t;
+    }
+  }
+  if (*T > tmin_all) *T = tmin_all;
+}
+
+/* The step restricted to a block of rows [m0, m1): for timing a bounded
+ * sample of a large grid.  Returns the rows' tmin. */
+double ora_step_rows(const ast_grid_t* g, const double* s, const double* u,
+                     double* s2, double* u2, const double* T_do, int method,
+                     int m0, int m1, int nthreads) {
+  ora_phys ph;
+  ora_phys_init(&ph, g->gamma, g->krit);
+  double tmin_all = 1e300;
+  const double tdo = *T_do;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(min : tmin_all)
+#endif
+  for (int m = m0; m < m1; m++) {
+    for (int n = 0; n < g->N; n++) {
+      double t = o_cell(g, &ph, (const d2*)s, (const d2*)u, (d2*)s2, (d2*)u2,
+                        tdo, method, n, m);
+      if (tmin_all > t) tmin_all = t;
+    }
+  }
+  return tmin_all;
+}
+
+/* funk_time, kernel.cu:1525-1536 (without the progress print). */
+void ora_funk_time(double* T, double* T_do, double* TT, int* i) {
+  *T_do = *T;
+  *TT = *TT + *T_do;
+  *T = 10000000;
+  *i = *i + 1;
+}
+
+int ora_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,70 @@
+/* asteriy_oracle.h -- CPU ORACLE (test infrastructure only; see
+ * asteriy_oracle.c).  Not part of the product: only tests/, the smoke check
+ * and the bench's CPU-baseline leg load it. */
+#ifndef ASTERIY_ORACLE_H
+#define ASTERIY_ORACLE_H
+#include "../include/asteriy.h"
+
+typedef struct ora_phys {
+  double ga, g1, g2, gp, gm, krit;
+} ora_phys;
+
+typedef struct ora_info {
+  int pattern;   /* -1 vacuum, 0 two rarefactions, 1 shock+rarefaction,
+                    2 two shocks, -9 none matched (NaN input)            */
+  int iters;     /* newton or devtwo iterations                           */
+  int capped;    /* AST_ITER_CAP reached                                  */
+  int swapped;   /* sides swapped so that pI <= pII (ipiz)                */
+  double p_star, u_star;
+  double enI, pI, rI, enII, pII, rII;   /* post-swap normal states        */
+} ora_info;
+
+void ora_phys_init(ora_phys* ph, double gamma, double krit);
+double ora_minmod(double x, double y);
+double ora_linear(double x1, double t1, double x2, double t2, double x3,
+                  double t3, double y);
+void ora_linear2(double x1, double t1, double x2, double t2, double x3,
+                 double t3, double y1, double y2, double* A, double* B);
+double ora_hll(const ora_phys*, const double* Ls, const double* Lu,
+               const double* Rs, const double* Ru, double n1, double n2,
+               double* Ps, double* Pu, double rad);
+double ora_hllc(const ora_phys*, const double* Ls, const double* Lu,
+                const double* Rs, const double* Ru, double n1, double n2,
+                double* Ps, double* Pu, double rad);
+double ora_godunov(const ora_phys*, const double* Ls, const double* Lu,
+                   const double* Rs, const double* Ru, double n1, double n2,
+                   double* Ps, double* Pu, double rad);
+double ora_godunov_ex(const ora_phys*, const double* Ls, const double* Lu,
+                      const double* Rs, const double* Ru, double n1, double n2,
+                      double* Ps, double* Pu, double rad, const double* guess,
+                      ora_info* info);
+double ora_flux(int method, const ora_phys*, const double* Ls,
+                const double* Lu, const double* Rs, const double* Ru,
+                double n1, double n2, double* Ps, double* Pu, double rad);
+void ora_flux_batch(int method, double gamma, double krit, long n,
+                    const double* in, double n1, double n2, double rad,
+                    double* out, int* iters, int nthreads);
+void ora_godunov_guess_batch(double gamma, double krit, long n, const double* in,
+                             double n1, double n2, double rad,
+                             const double* guess, double* out, int* iters,
+                             double* pstar);
+void ora_step(const ast_grid_t* g, const double* s, const double* u,
+              double* s2, double* u2, double* T, const double* T_do,
+              int method, int nthreads);
+double ora_step_rows(const ast_grid_t* g, const double* s, const double* u,
+                     double* s2, double* u2, const double* T_do, int method,
+                     int m0, int m1, int nthreads);
+void ora_funk_time(double* T, double* T_do, double* TT, int* i);
+int ora_max_threads(void);
+
+/* toro.c: independent exact Riemann solver (Toro, "Riemann Solvers and
+ * Numerical Methods for Fluid Dynamics", 3rd ed., ch. 4). */
+int toro_star(double gamma, double dl, double ul, double pl, double dr,
+              double ur, double pr, double* pm, double* um, int* iters);
+void toro_sample(double gamma, double dl, double ul, double pl, double dr,
+                 double ur, double pr, double pm, double um, double S,
+                 double* d, double* u, double* p);
+void toro_sod_profile(double gamma, double dl, double ul, double pl, double dr,
+                      double ur, double pr, double x0, double t, long n,
+                      const double* x, double* d, double* u, double* p);
+#endif

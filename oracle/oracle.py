@@ -1,0 +1,233 @@
+"""ctypes front-end of the CPU ORACLE (test infrastructure only).
+
+Loads oracle/liboracle.so (built by oracle/Makefile).  Only tests/, the smoke
+check and the bench's CPU-baseline leg import this module; the product never
+does.  Arrays are numpy float64; state planes are (M, N, 2) views of the
+reference's double2 planes s = {rho, p}, u = {vx, vy}.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class CPhys(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ga", "g1", "g2", "gp", "gm", "krit")]
+
+
+class CInfo(ctypes.Structure):
+    _fields_ = [("pattern", ctypes.c_int), ("iters", ctypes.c_int),
+                ("capped", ctypes.c_int), ("swapped", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("p_star", "u_star", "enI", "pI", "rI",
+                                        "enII", "pII", "rII")]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.ora_phys_init.argtypes = [ctypes.POINTER(CPhys), ctypes.c_double, ctypes.c_double]
+        L.ora_minmod.restype = ctypes.c_double
+        L.ora_minmod.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.ora_linear.restype = ctypes.c_double
+        L.ora_linear.argtypes = [ctypes.c_double] * 7
+        L.ora_linear2.argtypes = [ctypes.c_double] * 8 + [_dp, _dp]
+        for fn in ("ora_hll", "ora_hllc", "ora_godunov"):
+            f = getattr(L, fn)
+            f.restype = ctypes.c_double
+            f.argtypes = [ctypes.POINTER(CPhys), _dp, _dp, _dp, _dp, ctypes.c_double,
+                          ctypes.c_double, _dp, _dp, ctypes.c_double]
+        L.ora_godunov_ex.restype = ctypes.c_double
+        L.ora_godunov_ex.argtypes = [ctypes.POINTER(CPhys), _dp, _dp, _dp, _dp,
+                                     ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                     ctypes.c_double, _dp, ctypes.POINTER(CInfo)]
+        L.ora_flux_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_long, _dp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, _dp, _ip, ctypes.c_int]
+        L.ora_godunov_guess_batch.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_long,
+                                              _dp, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, _dp, _dp, _ip, _dp]
+        L.ora_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
+                               ctypes.c_int, ctypes.c_int]
+        L.ora_step_rows.restype = ctypes.c_double
+        L.ora_step_rows.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ora_funk_time.argtypes = [_dp, _dp, _dp, _ip]
+        L.ora_max_threads.restype = ctypes.c_int
+        L.toro_star.restype = ctypes.c_int
+        L.toro_star.argtypes = [ctypes.c_double] * 7 + [_dp, _dp, _ip]
+        L.toro_sample.argtypes = [ctypes.c_double] * 10 + [_dp, _dp, _dp]
+        L.toro_sod_profile.argtypes = [ctypes.c_double] * 9 + [ctypes.c_long, _dp, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def phys(gamma: float, krit: float) -> CPhys:
+    ph = CPhys()
+    lib().ora_phys_init(ctypes.byref(ph), gamma, krit)
+    return ph
+
+
+def minmod(a, b):
+    return lib().ora_minmod(a, b)
+
+
+def linear(x1, t1, x2, t2, x3, t3, y):
+    return lib().ora_linear(x1, t1, x2, t2, x3, t3, y)
+
+
+def linear2(x1, t1, x2, t2, x3, t3, y1, y2):
+    A, B = ctypes.c_double(), ctypes.c_double()
+    lib().ora_linear2(x1, t1, x2, t2, x3, t3, y1, y2, ctypes.byref(A), ctypes.byref(B))
+    return A.value, B.value
+
+
+_FLUX = {0: "ora_hll", 1: "ora_hllc", 2: "ora_godunov"}
+
+
+def flux(method: int, Ls, Lu, Rs, Ru, n1, n2, rad, gamma=5.0 / 3.0, krit=0.05):
+    """One face: returns (Ps, Pu, dt) as the reference operator does."""
+    ph = phys(gamma, krit)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (Ls, Lu, Rs, Ru)]
+    Ps, Pu = np.zeros(2), np.zeros(2)
+    t = getattr(lib(), _FLUX[method])(ctypes.byref(ph), *[_p(v) for v in a], n1, n2,
+                                       _p(Ps), _p(Pu), rad)
+    return Ps, Pu, t
+
+
+def godunov_info(Ls, Lu, Rs, Ru, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0, krit=0.05,
+                 guess=None):
+    ph = phys(gamma, krit)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (Ls, Lu, Rs, Ru)]
+    Ps, Pu = np.zeros(2), np.zeros(2)
+    info = CInfo()
+    g = None if guess is None else ctypes.byref(ctypes.c_double(guess))
+    t = lib().ora_godunov_ex(ctypes.byref(ph), *[_p(v) for v in a], n1, n2, _p(Ps), _p(Pu),
+                             rad, g, ctypes.byref(info))
+    return Ps, Pu, t, info
+
+
+def flux_batch(method: int, faces: np.ndarray, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0,
+               krit=0.05, nthreads: int = 0, want_iters: bool = False):
+    """faces: (8, n) SoA -> out (5, n) [, iters (n, 2) = (iterations, pattern)]."""
+    faces = np.ascontiguousarray(faces, dtype=np.float64)
+    n = faces.shape[1]
+    out = np.empty((5, n))
+    it = np.zeros((n, 2), dtype=np.int32) if want_iters else None
+    lib().ora_flux_batch(method, gamma, krit, n, _p(faces), n1, n2, rad, _p(out),
+                         it.ctypes.data_as(_ip) if it is not None else None, nthreads)
+    return (out, it) if want_iters else out
+
+
+def godunov_guess_batch(faces: np.ndarray, guess: np.ndarray | None, n1=1.0, n2=0.0,
+                        rad=1.0, gamma=5.0 / 3.0, krit=0.05):
+    """Exact solver with per-face Newton starting points (NaN = PVRS)."""
+    faces = np.ascontiguousarray(faces, dtype=np.float64)
+    n = faces.shape[1]
+    out = np.empty((5, n))
+    it = np.zeros((n, 2), dtype=np.int32)
+    ps = np.empty(n)
+    g = None
+    if guess is not None:
+        g = np.ascontiguousarray(guess, dtype=np.float64)
+    lib().ora_godunov_guess_batch(gamma, krit, n, _p(faces), n1, n2, rad,
+                                  _p(g) if g is not None else None, _p(out),
+                                  it.ctypes.data_as(_ip), _p(ps))
+    return out, it, ps
+
+
+def step(grid, s: np.ndarray, u: np.ndarray, T: float, T_do: float, method: int,
+         nthreads: int = 0):
+    """One Kernel_TVD step on the CPU.  s,u: (M,N,2) float64.  Returns
+    (s2, u2, T_new)."""
+    cg = grid.to_c()
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    s2, u2 = np.empty_like(s), np.empty_like(u)
+    Tv = np.array([T], dtype=np.float64)
+    Td = np.array([T_do], dtype=np.float64)
+    lib().ora_step(ctypes.byref(cg), _p(s), _p(u), _p(s2), _p(u2), _p(Tv), _p(Td), method,
+                   nthreads)
+    return s2, u2, float(Tv[0])
+
+
+def step_rows(grid, s, u, T_do, method, m0, m1, nthreads=0):
+    cg = grid.to_c()
+    s2, u2 = np.empty_like(s), np.empty_like(u)
+    Td = np.array([T_do], dtype=np.float64)
+    t = lib().ora_step_rows(ctypes.byref(cg), _p(s), _p(u), _p(s2), _p(u2), _p(Td), method,
+                            m0, m1, nthreads)
+    return s2, u2, t
+
+
+def funk_time(T, T_do, TT, i):
+    a = [np.array([v], dtype=np.float64) for v in (T, T_do, TT)]
+    iv = np.array([i], dtype=np.int32)
+    lib().ora_funk_time(*[_p(v) for v in a], iv.ctypes.data_as(_ip))
+    return float(a[0][0]), float(a[1][0]), float(a[2][0]), int(iv[0])
+
+
+class Sim:
+    """The reference's host protocol on the CPU (kernel.cu:3868-3906): step,
+    then funk_time; initial T = 1e7, T_do = 1e-9, TT = 0 (kernel.cu:3321-3324)."""
+
+    def __init__(self, grid, s, u, method, nthreads=0):
+        self.grid, self.method, self.nthreads = grid, method, nthreads
+        self.s = np.ascontiguousarray(s, dtype=np.float64)
+        self.u = np.ascontiguousarray(u, dtype=np.float64)
+        self.T, self.T_do, self.TT, self.i = 1e7, 1e-9, 0.0, 0
+
+    def advance(self, steps=1):
+        for _ in range(steps):
+            self.s, self.u, self.T = step(self.grid, self.s, self.u, self.T, self.T_do,
+                                          self.method, self.nthreads)
+            self.T, self.T_do, self.TT, self.i = funk_time(self.T, self.T_do, self.TT, self.i)
+        return self
+
+
+def toro_star(gamma, dl, ul, pl, dr, ur, pr):
+    pm, um, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    rc = lib().toro_star(gamma, dl, ul, pl, dr, ur, pr, ctypes.byref(pm), ctypes.byref(um),
+                         ctypes.byref(it))
+    if rc != 0:
+        return None
+    return pm.value, um.value
+
+
+def toro_sample(gamma, dl, ul, pl, dr, ur, pr, S):
+    st = toro_star(gamma, dl, ul, pl, dr, ur, pr)
+    d, u, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().toro_sample(gamma, dl, ul, pl, dr, ur, pr, st[0], st[1], S, ctypes.byref(d),
+                      ctypes.byref(u), ctypes.byref(p))
+    return d.value, u.value, p.value
+
+
+def toro_profile(gamma, left, right, x0, t, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    d, u, p = np.empty_like(x), np.empty_like(x), np.empty_like(x)
+    lib().toro_sod_profile(gamma, left[0], left[1], left[2], right[0], right[1], right[2],
+                           x0, t, x.size, _p(x), _p(d), _p(u), _p(p))
+    return d, u, p
