@@ -29,6 +29,7 @@ FAST = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 # (source, flag set)
 SOURCES = [
     ("api.cpp", PARITY),
+    ("sim.cpp", PARITY),
     ("misc.hip", PARITY),
     ("flux_f64.hip", PARITY),
     ("step_f64.hip", PARITY),

@@ -133,11 +133,10 @@ AST_DEV void block_min_to(double tmin, double* red, double* T) {
 
 // Adds the block's solver counters to *stats (device).
 template <int NT>
-AST_DEV void block_stats_to(const FaceStats& st, uint32_t fp64, uint32_t mlp, uint32_t* red,
-                            ast_stats_t* stats) {
+AST_DEV void block_stats_to(const FaceStats& st, uint32_t* red, ast_stats_t* stats) {
   constexpr int NW = NT / 64;
   uint32_t v[10] = {st.exact, st.newton, st.newton_it, st.bisect, st.bisect_it,
-                    st.raref, st.vacuum, st.capped, fp64, mlp};
+                    st.raref, st.vacuum, st.capped, st.fp64, st.mlp};
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int i = 0; i < 10; i++) {

@@ -1,0 +1,291 @@
+// exact_f32.hpp -- the exact Riemann face flux of the fast path.
+//
+// Godunov_Solver_Alexashov (kernel.cu:128-570) in FP32: the same side swap
+// (pI <= pII), the same wave-pattern test on Uud/Urz/Uvk, the same sampling
+// at the face (w = 0) with the reference's region edges, and the same flux
+// and dt -- with the p* solve adapted to FP32:
+//  * two shocks: the reference's newton (kernel.cu:635-680) in FP32, its
+//    stopping scale pee the PVRS-type estimate as in the reference, the
+//    tolerance raised from 1e-9 to 1e-6 (FP32 resolution) and the starting
+//    point optionally a learned guess;
+//  * shock + rarefaction: the root of lev (kernel.cu:751-765) on the same
+//    bracket [pI, pII] as devtwo's bisection (kernel.cu:682-749), found by a
+//    bracket-safeguarded Newton iteration to 1e-6 relative;
+//  * FP64 islands: the two-rarefaction closed form near vacuum
+//    ((Udf-Uvk)/(Urz-Uvk) < 0.05, where the FP32 difference cancels) and the
+//    shock + rarefaction solve in strong rarefactions (pI < 1e-4 pII) run in
+//    FP64; counted in ast_stats_t.fp64_faces.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "fast_flux.hpp"
+#include "ref_flux.hpp"
+
+namespace ast {
+
+struct ExactCtx {
+  float g, g1, g2, gp, gm, igm, ig1, c2g1, g1g2, c2g2;
+  double gd;
+  const float* mlp;  // learned-guess parameters in device memory, or null
+};
+
+static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
+  ExactCtx e;
+  const double g1 = gamma - 1.0, g2 = gamma + 1.0;
+  const double gp = (g2 / gamma) / 2.0, gm = (g1 / gamma) / 2.0;
+  e.g = (float)gamma;
+  e.g1 = (float)g1;
+  e.g2 = (float)g2;
+  e.gp = (float)gp;
+  e.gm = (float)gm;
+  e.igm = (float)(1.0 / gm);
+  e.ig1 = (float)(1.0 / g1);
+  e.c2g1 = (float)(2.0 / g1);
+  e.g1g2 = (float)(g1 / g2);
+  e.c2g2 = (float)(2.0 / g2);
+  e.gd = gamma;
+  e.mlp = (const float*)mlp;
+  return e;
+}
+
+// x^y for x > 0 via the hardware log2/exp2 (about 1 ulp of FP32 each)
+AST_DEV float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+
+constexpr float kNewtonTol32 = 1e-6f;
+constexpr float kVacuumIsland = 0.05f;
+constexpr float kRarefIsland = 1e-4f;
+
+// FP32 newton of kernel.cu:635-680 (two shocks), x^1.5 as x*sqrt(x).
+AST_DEV float newton_f32(const ExactCtx& e, float enI, float pI, float rI, float cI, float enII,
+                         float pII, float rII, float cII, float guess, bool use_guess,
+                         int& iters, bool& capped) {
+  const float g = e.g, gp = e.gp, gm = e.gm;
+  const float zI = rI * cI, zII = rII * cII;
+  float pn = (pI * zII + pII * zI + (enI - enII) * zI * zII) * frcp(zI + zII);
+  const float pee = pn;
+  if (use_guess) pn = guess;
+  const float ipI = frcp(pI), ipII = frcp(pII);
+  const float k1 = g + 1.f, k3 = 3.f * g - 1.f, k4 = 4.f * g;
+  int k = 0;
+  float p, dprev = INFINITY;
+  for (;;) {
+    p = pn;
+    k++;
+    const float aI = gp * p * ipI + gm, aII = gp * p * ipII + gm;
+    const float sI = fsqrt(aI), sII = fsqrt(aII);
+    const float fI = (p - pI) * frcp(zI * sI);
+    const float fII = (p - pII) * frcp(zII * sII);
+    const float dI = (k1 * p * ipI + k3) * frcp(k4 * zI * aI * sI);
+    const float dII = (k1 * p * ipII + k3) * frcp(k4 * zII * aII * sII);
+    pn = p - (fI + fII - (enI - enII)) * frcp(dI + dII);
+    // the reference's rule |pn - p| < tol*pee, floored at the FP32
+    // resolution of the residual (pee can lie far below p* in strong
+    // shocks, where the residual's rounding moves p by ~2 ulp), and
+    // stopped once the steps stop shrinking (the rounding floor)
+    const float d = fabsf(pn - p);
+    if (!(d >= fmaxf(kNewtonTol32 * pee, 2e-6f * fabsf(pn)))) break;
+    if (k > 2 && !(d < 0.5f * dprev)) break;
+    dprev = d;
+    if (k >= AST_ITER_CAP) { capped = true; break; }
+  }
+  iters = k;
+  return pn;
+}
+
+template <typename R> struct FastM;
+template <> struct FastM<float> {
+  static AST_DEV float sqrt_(float a) { return fsqrt(a); }
+  static AST_DEV float pow_(float a, float b) { return fpow(a, b); }
+  static AST_DEV float div_(float a, float b) { return a * frcp(b); }
+};
+template <> struct FastM<double> {
+  static AST_DEV double sqrt_(double a) { return ::sqrt(a); }
+  static AST_DEV double pow_(double a, double b) { return ::pow(a, b); }
+  static AST_DEV double div_(double a, double b) { return a / b; }
+};
+
+// Root of lev(p) (left shock + right rarefaction) on [pI, pII]:
+// safeguarded Newton.  R is float or double (FP64 island).
+template <typename R>
+AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R rII, R tol,
+                      int& iters, bool& capped) {
+  typedef FastM<R> F;
+  const R cI = F::sqrt_(F::div_(g * pI, rI)), cII = F::sqrt_(F::div_(g * pII, rII));
+  const R zI = rI * cI, zII = rII * cII, du = enI - enII, c2 = F::div_(R(2), g1) * cII;
+  const R ipI = F::div_(R(1), pI), ipII = F::div_(R(1), pII);
+  R lo = pI, hi = pII;
+  // start: the linearised (PVRS-type) estimate, clipped into the bracket
+  R p = F::div_(pI * zII + pII * zI + du * zI * zII, zI + zII);
+  if (!(p > lo && p < hi)) p = R(0.5) * (lo + hi);
+  int k = 0;
+  for (;;) {
+    k++;
+    const R a = gp * p * ipI + gm;
+    const R sa = F::sqrt_(a);
+    const R q = F::pow_(p * ipII, gm);
+    const R f = F::div_(p - pI, zI * sa) + c2 * (q - R(1)) - du;
+    const R df = F::div_((g + R(1)) * p * ipI + (R(3) * g - R(1)), R(4) * g * zI * a * sa) +
+                 F::div_(cII * q, g * p);
+    if (f < R(0)) lo = p;
+    else hi = p;
+    R pn = p - F::div_(f, df);
+    if (!(pn > lo && pn < hi)) pn = R(0.5) * (lo + hi);
+    const R dp = pn - p;
+    p = pn;
+    if (!((dp < R(0) ? -dp : dp) > tol * p) || !(hi - lo > tol * p)) break;
+    if (k >= AST_ITER_CAP) { capped = true; break; }
+  }
+  iters = k;
+  return p;
+}
+
+// The exact face flux in the face frame (u normal, v tangential); UU is the
+// largest |wave speed| used for dt.  `guess` (finite, > 0) seeds newton.
+AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
+                        FaceStats& st, float guess = -1.f, int* info = nullptr) {
+  const float g = e.g, g1 = e.g1, g2 = e.g2, gm = e.gm;
+  // swap sides so that pI <= pII, kernel.cu:156-184
+  const bool sw = L.p > R.p;
+  const float enI = sw ? -R.u : L.u, enII = sw ? -L.u : R.u;
+  const float teI = sw ? R.v : L.v, teII = sw ? L.v : R.v;
+  const float pI = sw ? R.p : L.p, pII = sw ? L.p : R.p;
+  const float rI = sw ? R.r : L.r, rII = sw ? L.r : R.r;
+  const float cI = fsqrt(g * pI * frcp(rI)), cII = fsqrt(g * pII * frcp(rII));
+  const float a = fsqrt(rI * (g2 * pII + g1 * pI) * 0.5f);
+  const float Uud = (pII - pI) * frcp(a);
+  const float Urz = -e.c2g1 * cII * (1.f - fpow(pI * frcp(pII), gm));
+  const float Uvk = -e.c2g1 * (cII + cI);
+  const float Udf = enI - enII;
+  st.exact += 1;
+  int pat;
+  float p = 0.f;
+  int it = 0;
+  bool capped = false;
+  if (Udf < Uvk) {
+    pat = -1;
+    st.vacuum += 1;
+  } else if (Udf <= Urz) {
+    pat = 0;
+    st.raref += 1;
+    const float t = (Udf - Uvk) * frcp(Urz - Uvk);
+    if (t < kVacuumIsland) {  // FP64 island: near vacuum
+      const double gd = e.gd, g1d = gd - 1.0, gmd = (g1d / gd) / 2.0;
+      const double cId = sqrt(gd * pI / (double)rI), cIId = sqrt(gd * pII / (double)rII);
+      const double Urzd = -2.0 * cIId / g1d * (1.0 - pow((double)pI / pII, gmd));
+      const double Uvkd = -2.0 * (cIId + cId) / g1d;
+      const double Udfd = (double)enI - (double)enII;
+      p = (float)(pI * pow((Udfd - Uvkd) / (Urzd - Uvkd), 1.0 / gmd));
+      st.fp64 += 1;
+    } else {
+      p = pI * fpow(t, e.igm);
+    }
+  } else if (Udf <= Uud) {
+    pat = 1;
+    st.bisect += 1;
+    if (pI < kRarefIsland * pII) {  // FP64 island: strong rarefaction
+      const double gd = e.gd;
+      p = (float)shock_raref<double>(gd, gd - 1.0, ((gd + 1.0) / gd) / 2.0, ((gd - 1.0) / gd) / 2.0,
+                                     enI, pI, rI, enII, pII, rII, 1e-9, it, capped);
+      st.fp64 += 1;
+    } else {
+      p = shock_raref<float>(g, g1, e.gp, gm, enI, pI, rI, enII, pII, rII, kNewtonTol32, it,
+                             capped);
+    }
+    st.bisect_it += it;
+  } else {
+    pat = 2;
+    st.newton += 1;
+    const bool ug = guess > 0.f && guess < 3.0e38f;
+    st.mlp += ug ? 1 : 0;
+    p = newton_f32(e, enI, pI, rI, cI, enII, pII, rII, cII, guess, ug, it, capped);
+    st.newton_it += it;
+  }
+  st.capped += capped ? 1 : 0;
+  if (info) {
+    const bool island = (pat == 0 && (Udf - Uvk) * frcp(Urz - Uvk) < kVacuumIsland) ||
+                        (pat == 1 && pI < kRarefIsland * pII);
+    info[0] = it;
+    info[1] = (pat + 1) | (capped ? 8 : 0) | (island ? 16 : 0);
+  }
+
+  // sample the solution at w = 0, kernel.cu:234-544
+  float en, pp, r, te;
+  if (pat == 2) {
+    const float aI = fsqrt(rI * (g2 * 0.5f * p + g1 * 0.5f * pI));
+    const float aII = fsqrt(rII * (g2 * 0.5f * p + g1 * 0.5f * pII));
+    const float u = (aI * enI + aII * enII + pI - pII) * frcp(aI + aII);
+    const float dI = enI - aI * frcp(rI), dII = enII + aII * frcp(rII);
+    UU = fmaxf(fabsf(dI), fabsf(dII));
+    if (0.f <= dI) { en = enI; pp = pI; r = rI; te = teI; }
+    else if (0.f <= u) { en = u; pp = p; r = rI * aI * frcp(aI - rI * (enI - u)); te = teI; }
+    else if (0.f < dII) { en = u; pp = p; r = rII * aII * frcp(aII + rII * (enII - u)); te = teII; }
+    else { en = enII; pp = pII; r = rII; te = teII; }
+  } else if (pat == 1) {
+    const float aI = fsqrt(rI * (g2 * 0.5f * p + g1 * 0.5f * pI));
+    float aII;
+    if (fabsf(p - pII) < 1e-6f * pII) aII = rII * cII;
+    else aII = gm * rII * cII * (1.f - p * frcp(pII)) * frcp(1.f - fpow(p * frcp(pII), gm));
+    const float u = (aI * enI + aII * enII + pI - pII) * frcp(aI + aII);
+    const float dI = enI - aI * frcp(rI), dII = enII + cII;
+    const float ddII = u + cII - g1 * (enII - u) * 0.5f;
+    UU = fmaxf(fmaxf(fabsf(dI), fabsf(dII)), fabsf(ddII));
+    if (0.f <= dI) { en = enI; pp = pI; r = rI; te = teI; }
+    else if (0.f <= u) { en = u; pp = p; r = rI * aI * frcp(aI - rI * (enI - u)); te = teI; }
+    else if (0.f <= ddII) {
+      const float ce = cII - g1 * 0.5f * (enII - u);
+      en = u; pp = p; r = g * p * frcp(ce * ce); te = teII;
+    } else if (0.f < dII) {
+      const float ce = -e.g1g2 * enII + e.c2g2 * cII;
+      en = -ce; pp = pII * fpow(ce * frcp(cII), e.igm); r = g * pp * frcp(ce * ce); te = teII;
+    } else { en = enII; pp = pII; r = rII; te = teII; }
+  } else if (pat == 0) {
+    float aI, aII;
+    if (fabsf(p - pI) < 1e-6f * pI) aI = rI * cI;
+    else aI = gm * rI * cI * (1.f - p * frcp(pI)) * frcp(1.f - fpow(p * frcp(pI), gm));
+    if (fabsf(p - pII) < 1e-6f * pII) aII = rII * cII;
+    else aII = gm * rII * cII * (1.f - p * frcp(pII)) * frcp(1.f - fpow(p * frcp(pII), gm));
+    const float u = (aI * enI + aII * enII + pI - pII) * frcp(aI + aII);
+    const float dI = enI - cI, ddI = u - cI - g1 * (enI - u) * 0.5f;
+    const float dII = enII + cII, ddII = u + cII - g1 * (enII - u) * 0.5f;
+    UU = fmaxf(fmaxf(fabsf(dI), fabsf(dII)), fmaxf(fabsf(ddII), fabsf(ddI)));
+    if (0.f <= dI) { en = enI; pp = pI; r = rI; te = teI; }
+    else if (0.f < ddI) {
+      const float ce = e.g1g2 * enI + e.c2g2 * cI;
+      en = ce; pp = pI * fpow(ce * frcp(cI), e.igm); r = g * pp * frcp(ce * ce); te = teI;
+    } else if (0.f <= u) {
+      const float ce = cI + g1 * 0.5f * (enI - u);
+      en = u; pp = p; r = g * p * frcp(ce * ce); te = teI;
+    } else if (0.f <= ddII) {
+      const float ce = cII - g1 * 0.5f * (enII - u);
+      en = u; pp = p; r = g * p * frcp(ce * ce); te = teII;
+    } else if (0.f < dII) {
+      const float ce = -e.g1g2 * enII + e.c2g2 * cII;
+      en = -ce; pp = pII * fpow(ce * frcp(cII), e.igm); r = g * pp * frcp(ce * ce); te = teII;
+    } else { en = enII; pp = pII; r = rII; te = teII; }
+  } else {  // vacuum, kernel.cu:480-544
+    const float dI = enI - cI, ddI = enI + e.c2g1 * cI;
+    const float dII = enII + cII, ddII = enII - e.c2g1 * cII;
+    UU = fmaxf(fmaxf(fabsf(dI), fabsf(dII)), fmaxf(fabsf(ddII), fabsf(ddI)));
+    if (0.f <= dI) { en = enI; pp = pI; r = rI; te = teI; }
+    else if (0.f < ddI) {
+      const float ce = e.g1g2 * enI + e.c2g2 * cI;
+      en = ce; pp = pI * fpow(ce * frcp(cI), e.igm); r = g * pp * frcp(ce * ce); te = teI;
+    } else if (0.f <= ddII) { en = 0.f; pp = 0.f; r = 0.f; te = 0.f; }
+    else if (0.f < dII) {
+      const float ce = -e.g1g2 * enII + e.c2g2 * cII;
+      en = -ce; pp = pII * fpow(ce * frcp(cII), e.igm); r = g * pp * frcp(ce * ce); te = teII;
+    } else { en = enII; pp = pII; r = rII; te = teII; }
+  }
+  if (!(UU > 1e-7f)) UU = 1e-7f;  // eps8 floor of the reference's dt, kernel.cu:246-253
+  if (sw) en = -en;
+  const float E = pp * e.ig1 + 0.5f * r * (en * en + te * te);
+  FluxF f;
+  f.m = r * en;
+  f.un = f.m * en + pp;
+  f.ut = f.m * te;
+  f.e = en * (E + pp);
+  return f;
+}
+
+}  // namespace ast

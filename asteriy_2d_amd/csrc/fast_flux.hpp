@@ -1,0 +1,100 @@
+// fast_flux.hpp -- FP32 face fluxes of the fast step path.
+//
+// The same HLL (kernel.cu:1399-1523) and HLLC_2d_Korolkov
+// (Gas_Dinamic.cu:1604-1826) algebra as the reference -- Davis wave-speed
+// estimates, supersonic upwinding with the reference's >= / <= edges, the
+// HLLC contact speed SM, the averaged star pressure ptz, tangential velocity
+// kept per side -- rewritten for throughput: states are given in the face
+// frame (u normal, v tangential), divisions become hardware reciprocals,
+// max(cL, cR) is taken before the square root (monotone), and the branchy
+// four-way choice of HLLC becomes selects so a wave never diverges.  The
+// return value is the largest |wave speed| of the face (the reference
+// returns krit*rad/that; the step turns the maximum into dt once).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ref_flux.hpp"
+
+namespace ast {
+
+struct PrimF {
+  float r, p, u, v;  // density, pressure, normal and tangential velocity
+};
+struct FluxF {
+  float m, e, un, ut;  // mass, energy, normal and tangential momentum
+};
+
+AST_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+AST_DEV float fsqrt(float x) { return __builtin_sqrtf(x); }
+
+// Physical flux of one side in the face frame.
+AST_DEV FluxF phys_flux(const PrimF& q, float E) {
+  FluxF f;
+  f.m = q.r * q.u;
+  f.un = f.m * q.u + q.p;
+  f.ut = f.m * q.v;
+  f.e = (E + q.p) * q.u;
+  return f;
+}
+
+// HLL with the reference's Davis speeds, kernel.cu:1436-1443, 1505-1519.
+AST_DEV FluxF hll_f32(const PrimF& L, const PrimF& R, float g, float ig1, float& UU) {
+  const float c2 = fmaxf(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
+  const float cm = fsqrt(c2);
+  const float DL = fminf(L.u, R.u) - cm;
+  const float DR = fmaxf(L.u, R.u) + cm;
+  UU = fmaxf(fabsf(DL), fabsf(DR));
+  const float EL = L.p * ig1 + 0.5f * L.r * (L.u * L.u + L.v * L.v);
+  const float ER = R.p * ig1 + 0.5f * R.r * (R.u * R.u + R.v * R.v);
+  const FluxF fl = phys_flux(L, EL), fr = phys_flux(R, ER);
+  const float id = frcp(DR - DL);
+  const float dd = DL * DR;
+  FluxF h;
+  h.m = (DR * fl.m - DL * fr.m + dd * (R.r - L.r)) * id;
+  h.un = (DR * fl.un - DL * fr.un + dd * (R.r * R.u - L.r * L.u)) * id;
+  h.ut = (DR * fl.ut - DL * fr.ut + dd * (R.r * R.v - L.r * L.v)) * id;
+  h.e = (DR * fl.e - DL * fr.e + dd * (ER - EL)) * id;
+  const bool upL = DL > 0.f, upR = DR < 0.f;
+  FluxF f;
+  f.m = upL ? fl.m : (upR ? fr.m : h.m);
+  f.un = upL ? fl.un : (upR ? fr.un : h.un);
+  f.ut = upL ? fl.ut : (upR ? fr.ut : h.ut);
+  f.e = upL ? fl.e : (upR ? fr.e : h.e);
+  return f;
+}
+
+// HLLC_2d_Korolkov_b_s (metod 2, W = 0, nul_potok = false).
+AST_DEV FluxF hllc_f32(const PrimF& L, const PrimF& R, float g, float ig1, float& UU) {
+  const float c2 = fmaxf(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
+  const float cm = fsqrt(c2);
+  const float SL = fminf(L.u, R.u) - cm;                 // Gas_Dinamic.cu:1636
+  const float SR = fmaxf(L.u, R.u) + cm;                 // Gas_Dinamic.cu:1637
+  UU = fmaxf(fabsf(SL), fabsf(SR));
+  const float suL = SL - L.u, suR = SR - R.u;
+  const float mL = L.r * suL, mR = R.r * suR;
+  const float SM = (mR * R.u - mL * L.u - R.p + L.p) * frcp(mR - mL);  // :1645-1646
+  // side whose flux (or star flux) is taken, Gas_Dinamic.cu:1686-1785
+  const bool supL = SL >= 0.f, supR = SR <= 0.f;
+  const bool left = supL || (!supR && SM >= 0.f);
+  const PrimF K = left ? L : R;
+  const float SK = left ? SL : SR;
+  const float suK = left ? suL : suR;
+  const float mK = left ? mL : mR;
+  const float EK = K.p * ig1 + 0.5f * K.r * (K.u * K.u + K.v * K.v);
+  FluxF f = phys_flux(K, EK);
+  // star state of side K, averaged ptz (Gas_Dinamic.cu:1739-1744, 1772-1802)
+  const float ptz = 0.5f * ((R.p + mR * (SM - R.u)) + (L.p + mL * (SM - L.u)));
+  const float inv = frcp(SK - SM);
+  const float rs = mK * inv;
+  const float Es = (EK * suK + ptz * SM - K.p * K.u) * inv;
+  const float dr = rs - K.r;
+  if (!(supL || supR)) {
+    f.m = f.m + SK * dr;
+    f.un = f.un + SK * (rs * SM - K.r * K.u);
+    f.ut = f.ut + SK * dr * K.v;
+    f.e = f.e + SK * (Es - EK);
+  }
+  return f;
+}
+
+}  // namespace ast

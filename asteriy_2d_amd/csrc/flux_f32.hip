@@ -1,0 +1,86 @@
+// flux_f32.hip -- batched FP32 face-flux operators over SoA face arrays:
+// HLL, HLLC and the exact solver (FP64 islands) of the fast path applied to
+// n independent faces with a common normal (n1, n2).  52 B/face of
+// algorithmic traffic (8 words in, 5 out).
+#include "common.hpp"
+#include "exact_f32.hpp"
+#include "fast_flux.hpp"
+
+namespace ast {
+namespace f32 {
+
+constexpr int FNT = 256;
+
+template <int METHOD>
+__global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __restrict__ in,
+                                                   float n1, float n2, float krad,
+                                                   ExactCtx ex, float* __restrict__ out,
+                                                   ast_stats_t* stats, int* __restrict__ info) {
+  __shared__ uint32_t redi[(FNT / 64) * 10];
+  FaceStats st;
+  st.clear();
+  const size_t stride = (size_t)gridDim.x * FNT;
+  for (size_t i = (size_t)blockIdx.x * FNT + threadIdx.x; i < n; i += stride) {
+    const float rl = in[i], pl = in[n + i], vxl = in[2 * n + i], vyl = in[3 * n + i];
+    const float rr = in[4 * n + i], pr = in[5 * n + i], vxr = in[6 * n + i], vyr = in[7 * n + i];
+    // face frame: normal (n1, n2), tangent (-n2, n1)
+    const PrimF L = {rl, pl, vxl * n1 + vyl * n2, -vxl * n2 + vyl * n1};
+    const PrimF R = {rr, pr, vxr * n1 + vyr * n2, -vxr * n2 + vyr * n1};
+    float UU;
+    FluxF f;
+    if constexpr (METHOD == AST_HLL) f = hll_f32(L, R, ex.g, ex.ig1, UU);
+    else if constexpr (METHOD == AST_HLLC) f = hllc_f32(L, R, ex.g, ex.ig1, UU);
+    else {
+      int inf[2];
+      f = exact_f32(L, R, ex, UU, st, -1.f, info ? inf : nullptr);
+      if (info) {
+        info[2 * i] = inf[0];
+        info[2 * i + 1] = inf[1];
+      }
+    }
+    out[i] = f.m;
+    out[n + i] = f.e;
+    out[2 * n + i] = f.un * n1 - f.ut * n2;
+    out[3 * n + i] = f.un * n2 + f.ut * n1;
+    out[4 * n + i] = krad * frcp(UU);
+  }
+  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<FNT>(st, redi, stats);
+}
+
+template <int METHOD>
+static int launch(size_t n, const float* in, float n1, float n2, float krad, const ExactCtx& ex,
+                  float* out, ast_stats_t* stats, int* info, hipStream_t stream) {
+  size_t blocks = (n + FNT - 1) / FNT;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  hipLaunchKernelGGL(flux_kernel<METHOD>, dim3((unsigned)blocks), dim3(FNT), 0, stream, n, in, n1,
+                     n2, krad, ex, out, stats, info);
+  AST_LAUNCH_CHECK("ast_flux_f32 launch");
+  return AST_OK;
+}
+
+}  // namespace f32
+}  // namespace ast
+
+extern "C" int ast_flux_f32_info(int method, size_t n, const float* in, float n1, float n2,
+                                 float rad, float gamma, float krit, float* out,
+                                 ast_stats_t* stats, int* info, void* stream) {
+  using namespace ast::f32;
+  if (n == 0) return AST_OK;
+  if (!in || !out) return ast_fail(AST_ERR_ARG, "ast_flux_f32: null in/out");
+  if (!(gamma > 1.0f)) return ast_fail(AST_ERR_ARG, "ast_flux_f32: gamma must be > 1");
+  const ast::ExactCtx ex = ast::make_exact_ctx((double)gamma, nullptr);
+  const float krad = krit * rad;
+  hipStream_t st = (hipStream_t)stream;
+  switch (method) {
+    case AST_HLL: return launch<AST_HLL>(n, in, n1, n2, krad, ex, out, stats, nullptr, st);
+    case AST_HLLC: return launch<AST_HLLC>(n, in, n1, n2, krad, ex, out, stats, nullptr, st);
+    case AST_EXACT: return launch<AST_EXACT>(n, in, n1, n2, krad, ex, out, stats, info, st);
+    default: return ast_fail(AST_ERR_ARG, "ast_flux_f32: unknown method %d", method);
+  }
+}
+
+extern "C" int ast_flux_f32(int method, size_t n, const float* in, float n1, float n2, float rad,
+                            float gamma, float krit, float* out, ast_stats_t* stats,
+                            void* stream) {
+  return ast_flux_f32_info(method, n, in, n1, n2, rad, gamma, krit, out, stats, nullptr, stream);
+}

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const double* __res
     out[3 * n + i] = Pu.y;
     out[4 * n + i] = t;
   }
-  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<FNT>(st, 0, 0, redi, stats);
+  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<FNT>(st, redi, stats);
 }
 
 template <int METHOD>

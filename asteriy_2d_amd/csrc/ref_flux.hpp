@@ -56,8 +56,10 @@ AST_DEV Phys<R> make_phys(R gamma, R krit) {
 
 // Per-face solver statistics kept in registers, summed per block.
 struct FaceStats {
-  uint32_t exact, newton, newton_it, bisect, bisect_it, raref, vacuum, capped;
-  AST_DEV void clear() { exact = newton = newton_it = bisect = bisect_it = raref = vacuum = capped = 0; }
+  uint32_t exact, newton, newton_it, bisect, bisect_it, raref, vacuum, capped, fp64, mlp;
+  AST_DEV void clear() {
+    exact = newton = newton_it = bisect = bisect_it = raref = vacuum = capped = fp64 = mlp = 0;
+  }
 };
 
 template <typename R> AST_DEV R my_min(R a, R b) { return (a <= b) ? a : b; }  // kernel.cu:53-63

@@ -1,0 +1,383 @@
+// sim.cpp -- the host runtime around the step: the reference's time loop
+// (kernel.cu:3868-3906: Kernel_TVD, funk_time, ping-pong by swapping the
+// plane arguments) as a native object.
+//
+// The reference launches, checks and device-synchronises after every kernel
+// (4 launches + 4 blocking syncs per 2 steps).  Here the two-step ping-pong
+// (step A->B, funk_time, step B->A, funk_time) is captured once into a HIP
+// graph on the simulation's own stream and replayed; dt never leaves the
+// device (T, T_do, TT, i live in one device block, as in kernel.cu:3290-3293).
+// State is held in FP64 (the reference layout) or FP32 planes.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "../../include/asteriy.h"
+
+int ast_fail(int code, const char* fmt, ...);
+int ast_hip_fail(hipError_t e, const char* where);
+int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
+                      double* T, const double* T_do, int method, const ast_mlp_t* mlp,
+                      ast_stats_t* stats, int krows, void* stream);
+extern "C" int ast_get_tile_rows(void);
+
+#define CK(call)                                              \
+  do {                                                        \
+    hipError_t _e = (call);                                   \
+    if (_e != hipSuccess) return ast_hip_fail(_e, #call);     \
+  } while (0)
+
+struct ast_sim {
+  ast_grid_t g;
+  int precision;  // 0: FP64 planes, 1: FP32 planes
+  int method;
+  int krows;
+  size_t cells;
+  void* plane[4];  // s0, u0, s1, u1
+  int cur;         // 0: state in (s0,u0); 1: in (s1,u1)
+  double* scal;    // T, T_do, TT (+ int i in the 4th word)
+  ast_stats_t* stats;
+  const ast_mlp_t* mlp;
+  hipStream_t stream;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  bool graph_ok;
+};
+typedef struct ast_sim ast_sim_t;
+
+static size_t elem(const ast_sim_t* s) { return s->precision ? sizeof(float) : sizeof(double); }
+
+static int launch_step(ast_sim_t* S, int from) {
+  void* s = S->plane[2 * from];
+  void* u = S->plane[2 * from + 1];
+  void* s2 = S->plane[2 * (1 - from)];
+  void* u2 = S->plane[2 * (1 - from) + 1];
+  double* T = S->scal;
+  int rc;
+  if (S->precision == 0)
+    rc = ast_step_f64(&S->g, (const double*)s, (const double*)u, (double*)s2, (double*)u2, T,
+                      T + 1, T + 2, S->method, S->stats, S->stream);
+  else
+    rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
+                           T + 1, S->method, S->mlp, S->stats, S->krows, S->stream);
+  if (rc != AST_OK) return rc;
+  return ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
+}
+
+static void drop_graph(ast_sim_t* S) {
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  S->exec = nullptr;
+  S->graph = nullptr;
+  S->graph_ok = false;
+}
+
+extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, ast_sim_t** out) {
+  if (!out) return ast_fail(AST_ERR_ARG, "ast_sim_create: null out");
+  *out = nullptr;
+  int rc = ast_grid_check(g);
+  if (rc != AST_OK) return rc;
+  if (precision != 0 && precision != 1)
+    return ast_fail(AST_ERR_ARG, "ast_sim_create: precision %d (0 = FP64, 1 = FP32)", precision);
+  if (method < AST_HLL || method > AST_EXACT)
+    return ast_fail(AST_ERR_ARG, "ast_sim_create: method %d", method);
+  if (ast_device_count() < 1) return ast_fail(AST_ERR_NOGPU, "ast_sim_create: no HIP device");
+  ast_sim_t* S = new (std::nothrow) ast_sim_t();
+  if (!S) return ast_fail(AST_ERR_SIZE, "ast_sim_create: out of host memory");
+  S->g = *g;
+  S->precision = precision;
+  S->method = method;
+  S->krows = ast_get_tile_rows();
+  S->cells = (size_t)g->N * g->M;
+  const size_t bytes = S->cells * 2 * elem(S);
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipMalloc(&S->plane[i], bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&S->scal, 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&S->stats, sizeof(ast_stats_t));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    ast_sim_t* tmp = S;
+    for (int i = 0; i < 4; i++) if (tmp->plane[i]) (void)hipFree(tmp->plane[i]);
+    if (tmp->scal) (void)hipFree(tmp->scal);
+    if (tmp->stats) (void)hipFree(tmp->stats);
+    delete tmp;
+    return ast_hip_fail(e, "ast_sim_create: allocation");
+  }
+  // kernel.cu:3321-3324: T = 1e7, T_do = 1e-9, TT = 0, i = 0
+  double init[4] = {1e7, 1e-9, 0.0, 0.0};
+  CK(hipMemcpy(S->scal, init, sizeof init, hipMemcpyHostToDevice));
+  CK(hipMemset(S->stats, 0, sizeof(ast_stats_t)));
+  *out = S;
+  return AST_OK;
+}
+
+extern "C" int ast_sim_destroy(ast_sim_t* S) {
+  if (!S) return AST_OK;
+  (void)hipStreamSynchronize(S->stream);
+  drop_graph(S);
+  for (int i = 0; i < 4; i++) (void)hipFree(S->plane[i]);
+  (void)hipFree(S->scal);
+  (void)hipFree(S->stats);
+  (void)hipStreamDestroy(S->stream);
+  delete S;
+  return AST_OK;
+}
+
+// Host state in the reference layout: s, u are N*M {x, y} pairs of doubles.
+extern "C" int ast_sim_set_state(ast_sim_t* S, const double* s, const double* u) {
+  if (!S || !s || !u) return ast_fail(AST_ERR_ARG, "ast_sim_set_state: null pointer");
+  const size_t n = S->cells * 2;
+  CK(hipStreamSynchronize(S->stream));
+  S->cur = 0;
+  if (S->precision == 0) {
+    CK(hipMemcpy(S->plane[0], s, n * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(S->plane[1], u, n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> a(n), b(n);
+    for (size_t k = 0; k < n; k++) {
+      a[k] = (float)s[k];
+      b[k] = (float)u[k];
+    }
+    CK(hipMemcpy(S->plane[0], a.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    CK(hipMemcpy(S->plane[1], b.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return AST_OK;
+}
+
+extern "C" int ast_sim_get_state(ast_sim_t* S, double* s, double* u) {
+  if (!S || !s || !u) return ast_fail(AST_ERR_ARG, "ast_sim_get_state: null pointer");
+  const size_t n = S->cells * 2;
+  CK(hipStreamSynchronize(S->stream));
+  void* ps = S->plane[2 * S->cur];
+  void* pu = S->plane[2 * S->cur + 1];
+  if (S->precision == 0) {
+    CK(hipMemcpy(s, ps, n * sizeof(double), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(u, pu, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> a(n), b(n);
+    CK(hipMemcpy(a.data(), ps, n * sizeof(float), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(b.data(), pu, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; k++) {
+      s[k] = a[k];
+      u[k] = b[k];
+    }
+  }
+  return AST_OK;
+}
+
+// Device planes holding the current state (for callers that keep data on
+// the GPU, e.g. error norms against another run).
+extern "C" int ast_sim_device_state(ast_sim_t* S, void** s, void** u) {
+  if (!S || !s || !u) return ast_fail(AST_ERR_ARG, "ast_sim_device_state: null pointer");
+  *s = S->plane[2 * S->cur];
+  *u = S->plane[2 * S->cur + 1];
+  return AST_OK;
+}
+
+extern "C" int ast_sim_set_time(ast_sim_t* S, double T, double T_do, double TT, int i) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_set_time: null sim");
+  double v[4] = {T, T_do, TT, 0.0};
+  memcpy(&v[3], &i, sizeof i);
+  CK(hipStreamSynchronize(S->stream));
+  CK(hipMemcpy(S->scal, v, sizeof v, hipMemcpyHostToDevice));
+  return AST_OK;
+}
+
+extern "C" int ast_sim_get_time(ast_sim_t* S, double* T, double* T_do, double* TT, int* i) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_get_time: null sim");
+  double v[4];
+  CK(hipStreamSynchronize(S->stream));
+  CK(hipMemcpy(v, S->scal, sizeof v, hipMemcpyDeviceToHost));
+  if (T) *T = v[0];
+  if (T_do) *T_do = v[1];
+  if (TT) *TT = v[2];
+  if (i) memcpy(i, &v[3], sizeof *i);
+  return AST_OK;
+}
+
+extern "C" int ast_sim_get_stats(ast_sim_t* S, ast_stats_t* out, int reset) {
+  if (!S || !out) return ast_fail(AST_ERR_ARG, "ast_sim_get_stats: null pointer");
+  CK(hipStreamSynchronize(S->stream));
+  CK(hipMemcpy(out, S->stats, sizeof *out, hipMemcpyDeviceToHost));
+  if (reset) CK(hipMemset(S->stats, 0, sizeof(ast_stats_t)));
+  return AST_OK;
+}
+
+extern "C" int ast_sim_set_mlp(ast_sim_t* S, const ast_mlp_t* mlp) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_set_mlp: null sim");
+  if (mlp && S->precision != 1)
+    return ast_fail(AST_ERR_ARG, "ast_sim_set_mlp: the learned guess serves the FP32 step");
+  CK(hipStreamSynchronize(S->stream));
+  S->mlp = mlp;
+  drop_graph(S);
+  return AST_OK;
+}
+
+// Rows per wave strip of the FP32 step (its tile shape; see the autotuner).
+extern "C" int ast_sim_set_tile_rows(ast_sim_t* S, int krows) {
+  if (!S || krows < 1 || krows > 1024) return ast_fail(AST_ERR_ARG, "ast_sim_set_tile_rows");
+  CK(hipStreamSynchronize(S->stream));
+  S->krows = krows;
+  drop_graph(S);
+  return AST_OK;
+}
+
+extern "C" void* ast_sim_stream(ast_sim_t* S) { return S ? (void*)S->stream : nullptr; }
+
+static int build_graph(ast_sim_t* S) {
+  drop_graph(S);
+  CK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_step(S, 0);
+  if (rc == AST_OK) rc = launch_step(S, 1);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(S->stream, &gr);
+  if (rc != AST_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return ast_hip_fail(e, "ast_sim_run: hipStreamEndCapture");
+  S->graph = gr;
+  CK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+  S->graph_ok = true;
+  return AST_OK;
+}
+
+// Advances `steps` steps asynchronously on the simulation stream.  With
+// use_graph, pairs of steps replay the captured graph (the graph always
+// starts from plane 0, so an odd position first takes one eager step).
+extern "C" int ast_sim_run(ast_sim_t* S, int steps, int use_graph) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_run: null sim");
+  if (steps < 0) return ast_fail(AST_ERR_ARG, "ast_sim_run: steps %d", steps);
+  int rc;
+  if (!use_graph) {
+    for (int k = 0; k < steps; k++) {
+      if ((rc = launch_step(S, S->cur)) != AST_OK) return rc;
+      S->cur ^= 1;
+    }
+    return AST_OK;
+  }
+  if (S->cur == 1 && steps > 0) {
+    if ((rc = launch_step(S, 1)) != AST_OK) return rc;
+    S->cur = 0;
+    steps--;
+  }
+  if (steps >= 2 && !S->graph_ok)
+    if ((rc = build_graph(S)) != AST_OK) return rc;
+  for (int k = 0; k + 1 < steps; k += 2) CK(hipGraphLaunch(S->exec, S->stream));
+  if (steps & 1) {
+    if ((rc = launch_step(S, 0)) != AST_OK) return rc;
+    S->cur = 1;
+  }
+  return AST_OK;
+}
+
+extern "C" int ast_sim_sync(ast_sim_t* S) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_sync: null sim");
+  CK(hipStreamSynchronize(S->stream));
+  return AST_OK;
+}
+
+// ---- on-disk formats ------------------------------------------------------
+
+// The reference's binary state file (B19.bin read at kernel.cu:3414-3436,
+// B31.bin written at kernel.cu:4111-4148): per cell in k order, 6 doubles
+// x, y, rho, p, vx, vy.  It holds no time: a resume from it restarts with
+// T_do = 1e-9 as the reference does.
+extern "C" int ast_sim_write_b31(ast_sim_t* S, const char* path) {
+  if (!S || !path) return ast_fail(AST_ERR_ARG, "ast_sim_write_b31: null argument");
+  std::vector<double> s(S->cells * 2), u(S->cells * 2);
+  int rc = ast_sim_get_state(S, s.data(), u.data());
+  if (rc != AST_OK) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return ast_fail(AST_ERR_ARG, "ast_sim_write_b31: cannot open %s", path);
+  for (size_t k = 0; k < S->cells; k++) {
+    const int n = (int)(k % S->g.N), m = (int)(k / S->g.N);
+    double rec[6] = {S->g.x_min + n * S->g.dx, S->g.y_min + m * S->g.dy, s[2 * k], s[2 * k + 1],
+                     u[2 * k], u[2 * k + 1]};
+    if (fwrite(rec, sizeof rec, 1, f) != 1) {
+      fclose(f);
+      return ast_fail(AST_ERR_ARG, "ast_sim_write_b31: write failed");
+    }
+  }
+  fclose(f);
+  return AST_OK;
+}
+
+extern "C" int ast_sim_read_b31(ast_sim_t* S, const char* path) {
+  if (!S || !path) return ast_fail(AST_ERR_ARG, "ast_sim_read_b31: null argument");
+  FILE* f = fopen(path, "rb");
+  if (!f) return ast_fail(AST_ERR_ARG, "ast_sim_read_b31: cannot open %s", path);
+  std::vector<double> s(S->cells * 2), u(S->cells * 2);
+  for (size_t k = 0; k < S->cells; k++) {
+    double rec[6];
+    if (fread(rec, sizeof rec, 1, f) != 1) {
+      fclose(f);
+      return ast_fail(AST_ERR_ARG, "ast_sim_read_b31: %s is shorter than N*M records", path);
+    }
+    s[2 * k] = rec[2];
+    s[2 * k + 1] = rec[3];
+    u[2 * k] = rec[4];
+    u[2 * k + 1] = rec[5];
+  }
+  fclose(f);
+  return ast_sim_set_state(S, s.data(), u.data());
+}
+
+// Checkpoint with the time state the reference does not save: header
+// {magic, version, N, M, precision, i, T, T_do, TT} + raw device planes.
+// A run resumed from it continues bit for bit.
+struct CkptHeader {
+  char magic[8];
+  int32_t version, N, M, precision, i, pad;
+  double T, T_do, TT;
+};
+
+extern "C" int ast_sim_save(ast_sim_t* S, const char* path) {
+  if (!S || !path) return ast_fail(AST_ERR_ARG, "ast_sim_save: null argument");
+  CkptHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, "ASTCKPT", 8);
+  h.version = 1;
+  h.N = S->g.N;
+  h.M = S->g.M;
+  h.precision = S->precision;
+  int rc = ast_sim_get_time(S, &h.T, &h.T_do, &h.TT, &h.i);
+  if (rc != AST_OK) return rc;
+  const size_t bytes = S->cells * 2 * elem(S);
+  std::vector<char> a(bytes), b(bytes);
+  CK(hipMemcpy(a.data(), S->plane[2 * S->cur], bytes, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(b.data(), S->plane[2 * S->cur + 1], bytes, hipMemcpyDeviceToHost));
+  FILE* f = fopen(path, "wb");
+  if (!f) return ast_fail(AST_ERR_ARG, "ast_sim_save: cannot open %s", path);
+  bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(a.data(), 1, bytes, f) == bytes &&
+            fwrite(b.data(), 1, bytes, f) == bytes;
+  fclose(f);
+  return ok ? AST_OK : ast_fail(AST_ERR_ARG, "ast_sim_save: write failed");
+}
+
+extern "C" int ast_sim_load(ast_sim_t* S, const char* path) {
+  if (!S || !path) return ast_fail(AST_ERR_ARG, "ast_sim_load: null argument");
+  FILE* f = fopen(path, "rb");
+  if (!f) return ast_fail(AST_ERR_ARG, "ast_sim_load: cannot open %s", path);
+  CkptHeader h;
+  const size_t bytes = S->cells * 2 * elem(S);
+  std::vector<char> a(bytes), b(bytes);
+  bool ok = fread(&h, sizeof h, 1, f) == 1;
+  if (ok && (memcmp(h.magic, "ASTCKPT", 8) != 0 || h.N != S->g.N || h.M != S->g.M ||
+             h.precision != S->precision)) {
+    fclose(f);
+    return ast_fail(AST_ERR_ARG, "ast_sim_load: %s does not match this simulation", path);
+  }
+  ok = ok && fread(a.data(), 1, bytes, f) == bytes && fread(b.data(), 1, bytes, f) == bytes;
+  fclose(f);
+  if (!ok) return ast_fail(AST_ERR_ARG, "ast_sim_load: %s is truncated", path);
+  CK(hipStreamSynchronize(S->stream));
+  S->cur = 0;
+  CK(hipMemcpy(S->plane[0], a.data(), bytes, hipMemcpyHostToDevice));
+  CK(hipMemcpy(S->plane[1], b.data(), bytes, hipMemcpyHostToDevice));
+  return ast_sim_set_time(S, h.T, h.T_do, h.TT, h.i);
+}

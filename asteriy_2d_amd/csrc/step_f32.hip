@@ -1,0 +1,342 @@
+// step_f32.hip -- the fast step: FP32 state, every face evaluated once.
+//
+// Same scheme as the reference step (Kernel_TVD, kernel.cu:2513-3073): minmod
+// piecewise-linear reconstruction with the positivity fallback, a Riemann
+// flux per face, the planar or axisymmetric (KorolkovSolver.cu:13-32) update
+// and the CFL dt -- reorganised for gfx950:
+//
+//  * "wave strip": a wave owns a band of 64 columns (60 output columns plus
+//    2 halo columns on each side) and walks down KR rows.  The x-neighbours
+//    of a cell live in the adjacent lanes and are read with DPP wave shifts
+//    (wave_shr:1 / wave_shl:1, a register move, no LDS); the y-neighbours
+//    are a 3-row sliding window in registers.  Each lane evaluates one
+//    x-face (its west face) and one y-face (its north face) per row, and
+//    the east face flux arrives from the next lane by another DPP shift, so
+//    every face of the grid is computed once (plus the strip/band seams).
+//  * state traffic: each cell is read once per strip (+ 4 halo rows) and
+//    written once: 16 B in + 16 B out per cell-update in FP32.
+//  * slopes use the minmod of neighbour differences (the reference divides
+//    by coordinate differences and multiplies by the half spacing: the
+//    same number in exact arithmetic, kernel.cu:35-50).
+//  * dt: each lane keeps the largest |wave speed| of its x- and y-faces;
+//    dt = krit*min(dx/max_x, dy/max_y) in FP64, reduced by wave shuffles and
+//    one 64-bit atomicMin per workgroup (deterministic).
+//  * the exact solver runs in FP32 with its p* solve promoted to FP64 near
+//    vacuum and in strong rarefactions, optionally seeded by the learned
+//    guess (exact_f32.hpp).
+#include "common.hpp"
+#include "exact_f32.hpp"
+#include "fast_flux.hpp"
+
+namespace ast {
+namespace f32 {
+
+constexpr int WOUT = 60;  // output columns per wave
+constexpr int WPB = 4;    // waves per workgroup (stacked strips)
+constexpr int NT = 64 * WPB;
+
+struct Q {
+  float r, p, vx, vy;
+};
+
+// lane i <- lane i-1 (lane 0 keeps its own value)
+AST_DEV float shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
+                                                    0xf, 0xf, false));
+}
+// lane i <- lane i+1 (lane 63 keeps its own value)
+AST_DEV float shl1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
+                                                    0xf, 0xf, false));
+}
+AST_DEV Q shr1(const Q& q) { return Q{shr1(q.r), shr1(q.p), shr1(q.vx), shr1(q.vy)}; }
+AST_DEV Q shl1(const Q& q) { return Q{shl1(q.r), shl1(q.p), shl1(q.vx), shl1(q.vy)}; }
+AST_DEV FluxF shl1(const FluxF& f) { return FluxF{shl1(f.m), shl1(f.e), shl1(f.un), shl1(f.ut)}; }
+
+// half the minmod slope, 0.5*minmod(a, b) (kernel.cu:22-33): minmod(a, b) is
+// the median of {a, b, 0} -- one v_med3_f32.
+AST_DEV float hmm(float a, float b) { return 0.5f * __builtin_amdgcn_fmed3f(a, b, 0.f); }
+
+// Face values at +-1/2 of a cell from its half slopes, with the positivity
+// fallback of kernel.cu:2706-2788 (rho, p <= 0 -> the cell value).
+AST_DEV void face_values(const Q& c, const Q& h, Q& plus, Q& minus) {
+  plus.r = c.r + h.r;
+  minus.r = c.r - h.r;
+  plus.p = c.p + h.p;
+  minus.p = c.p - h.p;
+  plus.vx = c.vx + h.vx;
+  minus.vx = c.vx - h.vx;
+  plus.vy = c.vy + h.vy;
+  minus.vy = c.vy - h.vy;
+  if (plus.r <= 0.f) plus.r = c.r;
+  if (minus.r <= 0.f) minus.r = c.r;
+  if (plus.p <= 0.f) plus.p = c.p;
+  if (minus.p <= 0.f) minus.p = c.p;
+}
+
+AST_DEV Q half_slopes(const Q& a, const Q& c, const Q& b) {
+  return Q{hmm(c.r - a.r, b.r - c.r), hmm(c.p - a.p, b.p - c.p), hmm(c.vx - a.vx, b.vx - c.vx),
+           hmm(c.vy - a.vy, b.vy - c.vy)};
+}
+
+// Cell (n, m) in FP32 with the ghost rules; corner/out-of-halo positions are
+// clamped into the grid (never read by an output cell, kept finite).
+AST_DEV Q load(const ast_grid_t& g, const float2* __restrict__ s, const float2* __restrict__ u,
+               int n, int m) {
+  const bool xin = (n >= 0) & (n < g.N), yin = (m >= 0) & (m < g.M);
+  float2 a, b;
+  if ((!xin & !yin) | (n < -2) | (n > g.N + 1) | (m < -2) | (m > g.M + 1)) {
+    const int cn = n < 0 ? 0 : (n >= g.N ? g.N - 1 : n);
+    const int cm = m < 0 ? 0 : (m >= g.M ? g.M - 1 : m);
+    const size_t k = (size_t)cm * g.N + cn;
+    a = s[k];
+    b = u[k];
+  } else {
+    fetch_cell(g, s, u, n, m, a, b);
+  }
+  return Q{a.x, a.y, b.x, b.y};
+}
+
+struct Consts {
+  float g, ig1, g1, gig1;  // gamma, 1/(gamma-1), gamma-1, gamma/(gamma-1)
+  float fx0, iax2, iay2;   // frozen ellipse
+  int frozen;
+};
+
+template <int METHOD>
+AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
+                        const ExactCtx& ex, FaceStats& st) {
+  if constexpr (METHOD == AST_HLL) return hll_f32(L, R, c.g, c.ig1, UU);
+  else if constexpr (METHOD == AST_HLLC) return hllc_f32(L, R, c.g, c.ig1, UU);
+  else return exact_f32(L, R, ex, UU, st);
+}
+
+// One face in the x (NORMAL_X) or y direction: rotate to the face frame,
+// flux, and back.  With WINDOW, faces outside the high-order window use
+// HLL on the cell states (kernel.cu:2812, 2838-2839, 2856).
+template <int METHOD, bool WINDOW, bool NORMAL_X>
+AST_DEV FluxF face(const Q& l, const Q& r, const Q& lc, const Q& rc, bool ho, const Consts& c,
+                   float& UU, const ExactCtx& ex, FaceStats& st) {
+  PrimF L, R;
+  if (WINDOW && !ho) {
+    L = NORMAL_X ? PrimF{lc.r, lc.p, lc.vx, lc.vy} : PrimF{lc.r, lc.p, lc.vy, lc.vx};
+    R = NORMAL_X ? PrimF{rc.r, rc.p, rc.vx, rc.vy} : PrimF{rc.r, rc.p, rc.vy, rc.vx};
+    return hll_f32(L, R, c.g, c.ig1, UU);
+  }
+  L = NORMAL_X ? PrimF{l.r, l.p, l.vx, l.vy} : PrimF{l.r, l.p, l.vy, l.vx};
+  R = NORMAL_X ? PrimF{r.r, r.p, r.vx, r.vy} : PrimF{r.r, r.p, r.vy, r.vx};
+  return face_flux<METHOD>(L, R, c, UU, ex, st);
+}
+
+AST_DEV bool frozen_at(const Consts& c, float x, float y) {
+  if (!c.frozen) return false;
+  const float d = x - c.fx0;
+  return d * d * c.iax2 + y * y * c.iay2 < 1.f;
+}
+
+template <int METHOD, int GEOM, bool WINDOW>
+__global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactCtx ex,
+                                                  const float2* __restrict__ s,
+                                                  const float2* __restrict__ u,
+                                                  float2* __restrict__ s2,
+                                                  float2* __restrict__ u2, double* T,
+                                                  const double* __restrict__ T_do,
+                                                  ast_stats_t* stats, int nbands, int krows,
+                                                  int nwg) {
+  __shared__ double red[WPB];
+  __shared__ uint32_t redi[WPB * 10];
+  const int wg = xcd_tile(blockIdx.x, nwg);
+  const int band = wg % nbands, sgrp = wg / nbands;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = (sgrp * WPB + wave) * krows;
+  const int m1 = min(m0 + krows, g.M);
+  const int n = band * WOUT + lane - 2;
+  const bool out_col = (lane >= 2) & (lane < 2 + WOUT) & (n < g.N);
+  const bool xvote = (lane >= 2) & (lane <= 2 + WOUT) & (n <= g.N);
+  const double tdo = *T_do;
+  const double dV = g.dx * g.dy;
+  const float tdV = (float)(tdo / dV), tdof = (float)tdo;
+  const float dxf = (float)g.dx, dyf = (float)g.dy;
+  const float x = (float)(g.x_min + n * g.dx);
+  const bool ho_x = high_order(g, g.x_min + n * g.dx - 0.5 * g.dx);  // west face window
+  const bool ho_y = high_order(g, g.x_min + n * g.dx);               // y faces: cell x
+  FaceStats st;
+  st.clear();
+  float UUx = 0.f, UUy = 0.f;
+
+  if (m0 < g.M) {
+    // prologue: rows m0-2 .. m0+1, the south face of row m0
+    Q qm2 = load(g, s, u, n, m0 - 2);
+    Q qm1 = load(g, s, u, n, m0 - 1);
+    Q q0 = load(g, s, u, n, m0);
+    Q q1 = load(g, s, u, n, m0 + 1);
+    Q nvm1, svm1, nv0, sv0;
+    face_values(qm1, half_slopes(qm2, qm1, q0), nvm1, svm1);
+    face_values(q0, half_slopes(qm1, q0, q1), nv0, sv0);
+    float uu;
+    FluxF Gs = face<METHOD, WINDOW, false>(nvm1, sv0, qm1, q0, ho_y, c, uu, ex, st);
+    bool fr0 = frozen_at(c, x, (float)(g.y_min + m0 * g.dy));
+    bool frm1 = frozen_at(c, x, (float)(g.y_min + (m0 - 1) * g.dy));
+    if (out_col && !(fr0 && frm1)) UUy = fmaxf(UUy, uu);
+
+    // rows are fetched one iteration ahead of their use (latency hiding)
+    Q qnext = load(g, s, u, n, m0 + 2);
+    for (int m = m0; m < m1; m++) {
+      const Q q2 = qnext;
+      if (m + 1 < m1) qnext = load(g, s, u, n, m + 3);
+      Q nv1, sv1;
+      face_values(q1, half_slopes(q0, q1, q2), nv1, sv1);
+      const float y = (float)(g.y_min + m * g.dy);
+      const bool fr1 = frozen_at(c, x, (float)(g.y_min + (m + 1) * g.dy));
+      // north face of row m
+      const FluxF Gn = face<METHOD, WINDOW, false>(nv0, sv1, q0, q1, ho_y, c, uu, ex, st);
+      if (out_col && !(fr0 && fr1)) UUy = fmaxf(UUy, uu);
+      // west face of row m: left state = east value of lane-1
+      const Q ql = shr1(q0), qr = shl1(q0);
+      Q ev, wv;
+      face_values(q0, half_slopes(ql, q0, qr), ev, wv);
+      const Q evl = shr1(ev);
+      float ux;
+      const FluxF Fw = face<METHOD, WINDOW, true>(evl, wv, ql, q0, ho_x, c, ux, ex, st);
+      const FluxF Fe = shl1(Fw);
+      const bool frl = frozen_at(c, x - dxf, y);
+      if (xvote && !(fr0 && frl)) UUx = fmaxf(UUx, ux);
+
+      if (out_col) {
+        float r2, p2, vx2, vy2;
+        if (fr0) {  // kernel.cu:2317-2324
+          r2 = q0.r; p2 = q0.p; vx2 = q0.vx; vy2 = q0.vy;
+        } else {
+          // outward flux sums, kernel.cu:3027-3032 (x faces: Fe - Fw, y faces: Gn - Gs)
+          const float PSx = (Fe.m - Fw.m) * dyf + (Gn.m - Gs.m) * dxf;
+          const float PSy = (Fe.e - Fw.e) * dyf + (Gn.e - Gs.e) * dxf;
+          const float PUx = (Fe.un - Fw.un) * dyf + (Gn.ut - Gs.ut) * dxf;
+          const float PUy = (Fe.ut - Fw.ut) * dyf + (Gn.un - Gs.un) * dxf;
+          const float ke = 0.5f * q0.r * (q0.vx * q0.vx + q0.vy * q0.vy);
+          float dr = -tdV * PSx, dmx = -tdV * PUx, dmy = -tdV * PUy, dE = -tdV * PSy;
+          if constexpr (GEOM == AST_AXISYMMETRIC) {
+            // KorolkovSolver.cu:13-32: -(T_do/y) v {rho, rho u, rho v, E + p}
+            const float ty = tdof / y;
+            const float rv = q0.r * q0.vy;
+            dr -= ty * rv;
+            dmx -= ty * rv * q0.vx;
+            dmy -= ty * rv * q0.vy;
+            dE -= ty * q0.vy * (c.gig1 * q0.p + ke);
+          }
+          r2 = q0.r + dr;
+          if (r2 <= 0.f) r2 = q0.r;  // kernel.cu:3060-3064
+          const float ir = frcp(r2);
+          vx2 = (q0.r * q0.vx + dmx) * ir;
+          vy2 = (q0.r * q0.vy + dmy) * ir;
+          // p' = (E' - rho'|u'|^2/2)(gamma-1), written so that the O(E)
+          // terms cancel exactly: E' - KE' = p/(g-1) + dE - (KE' - KE) with
+          // KE' - KE = (m.dm + |dm|^2/2 - KE*dr) / rho'.
+          const float mx = q0.r * q0.vx, my = q0.r * q0.vy;
+          const float dke = (mx * dmx + my * dmy + 0.5f * (dmx * dmx + dmy * dmy) - ke * dr) * ir;
+          p2 = q0.p + c.g1 * (dE - dke);
+          if (p2 <= 0.f) p2 = 0.000001f;  // kernel.cu:3070-3073
+        }
+        const size_t k = (size_t)m * g.N + n;
+        s2[k] = make_float2(r2, p2);
+        u2[k] = make_float2(vx2, vy2);
+      }
+      // slide the window
+      q0 = q1;
+      q1 = q2;
+      nv0 = nv1;
+      Gs = Gn;
+      fr0 = fr1;
+    }
+  }
+  // per-face dt = krit*rad/UU (monotone in UU), capped at 1000 (kernel.cu:2530)
+  double tmin = (out_col && m0 < g.M) ? 1000.0 : INFINITY;
+  if (UUx > 0.f) tmin = fmin(tmin, g.krit * g.dx / (double)UUx);
+  if (UUy > 0.f) tmin = fmin(tmin, g.krit * g.dy / (double)UUy);
+  block_min_to<NT>(tmin, red, T);
+  if (METHOD == AST_EXACT && stats != nullptr)
+    block_stats_to<NT>(st, redi, stats);
+}
+
+}  // namespace f32
+}  // namespace ast
+
+using namespace ast;
+using namespace ast::f32;
+
+static int g_krows = 8;
+
+extern "C" int ast_set_tile_rows(int krows) {
+  if (krows < 1 || krows > 1024) return ast_fail(AST_ERR_ARG, "ast_set_tile_rows: %d", krows);
+  g_krows = krows;
+  return AST_OK;
+}
+extern "C" int ast_get_tile_rows(void) { return g_krows; }
+
+template <int METHOD, int GEOM, bool WINDOW>
+static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
+                      const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
+                      ast_stats_t* stats, int krows, hipStream_t stream) {
+  const int nbands = (g.N + WOUT - 1) / WOUT;
+  const int nstrips = (g.M + krows - 1) / krows;
+  const int nsg = (nstrips + WPB - 1) / WPB;
+  const long long nwg = (long long)nbands * nsg;
+  if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
+  hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW>), dim3((unsigned)nwg), dim3(NT), 0,
+                     stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg);
+  AST_LAUNCH_CHECK("ast_step_f32 launch");
+  return AST_OK;
+}
+
+template <int METHOD>
+static int dispatch_geom(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, bool window,
+                         const float2* s, const float2* u, float2* s2, float2* u2, double* T,
+                         const double* T_do, ast_stats_t* stats, int krows, hipStream_t st) {
+  const bool axi = g.geometry == AST_AXISYMMETRIC;
+  if (axi) {
+    return window ? launch_f32<METHOD, AST_AXISYMMETRIC, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st)
+                  : launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st);
+  }
+  return window ? launch_f32<METHOD, AST_PLANAR, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st)
+                : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st);
+}
+
+int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
+                      double* T, const double* T_do, int method, const ast_mlp_t* mlp,
+                      ast_stats_t* stats, int krows, void* stream) {
+  int rc = ast_grid_check(g);
+  if (rc != AST_OK) return rc;
+  if (!s || !u || !s2 || !u2 || !T || !T_do)
+    return ast_fail(AST_ERR_ARG, "ast_step_f32: null state or time pointer");
+  if (s == s2 || u == u2 || s == u2 || u == s2)
+    return ast_fail(AST_ERR_ARG, "ast_step_f32: output planes must not alias the input planes");
+  if (krows < 1) return ast_fail(AST_ERR_ARG, "ast_step_f32: krows %d", krows);
+  Consts c;
+  c.g = (float)g->gamma;
+  c.g1 = (float)(g->gamma - 1.0);
+  c.ig1 = (float)(1.0 / (g->gamma - 1.0));
+  c.gig1 = (float)(g->gamma / (g->gamma - 1.0));
+  c.frozen = g->frozen_ax > 0.0;
+  c.fx0 = (float)g->frozen_x0;
+  c.iax2 = c.frozen ? (float)(1.0 / (g->frozen_ax * g->frozen_ax)) : 0.f;
+  c.iay2 = c.frozen ? (float)(1.0 / (g->frozen_ay * g->frozen_ay)) : 0.f;
+  const ExactCtx ex = make_exact_ctx(g->gamma, mlp);
+  const bool window = !(g->ho_lo == -INFINITY && g->ho_hi == INFINITY);
+  hipStream_t st = (hipStream_t)stream;
+  const float2* S = (const float2*)s;
+  const float2* U = (const float2*)u;
+  float2* S2 = (float2*)s2;
+  float2* U2 = (float2*)u2;
+  switch (method) {
+    case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, st);
+    case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, st);
+    case AST_EXACT: return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, st);
+    default: return ast_fail(AST_ERR_ARG, "ast_step_f32: unknown method %d", method);
+  }
+}
+
+extern "C" int ast_step_f32(const ast_grid_t* g, const float* s, const float* u, float* s2,
+                            float* u2, double* T, const double* T_do, double* TT, int method,
+                            const ast_mlp_t* mlp, ast_stats_t* stats, void* stream) {
+  (void)TT;
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, stream);
+}

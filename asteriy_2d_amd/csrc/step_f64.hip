@@ -218,7 +218,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, const double2* _
     u2[k] = make_double2(uo.x, uo.y);
   }
   block_min_to<NT>(tmin, red, T);
-  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<NT>(st, 0, 0, redi, stats);
+  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<NT>(st, redi, stats);
 }
 
 template <int METHOD, int GEOM>
