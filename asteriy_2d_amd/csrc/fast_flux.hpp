@@ -25,7 +25,7 @@ struct FluxF {
 };
 
 AST_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-AST_DEV float fsqrt(float x) { return __builtin_sqrtf(x); }
+AST_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Physical flux of one side in the face frame.
 AST_DEV FluxF phys_flux(const PrimF& q, float E) {
@@ -88,12 +88,14 @@ AST_DEV FluxF hllc_f32(const PrimF& L, const PrimF& R, float g, float ig1, float
   const float rs = mK * inv;
   const float Es = (EK * suK + ptz * SM - K.p * K.u) * inv;
   const float dr = rs - K.r;
-  if (!(supL || supR)) {
-    f.m = f.m + SK * dr;
-    f.un = f.un + SK * (rs * SM - K.r * K.u);
-    f.ut = f.ut + SK * dr * K.v;
-    f.e = f.e + SK * (Es - EK);
-  }
+  // subsonic: F*_K = F_K + S_K (U*_K - U_K); supersonic: F_K (selects, no
+  // branch -- the star terms may be non-finite there and are discarded)
+  const bool sub = !(supL || supR);
+  const float sk = SK;
+  f.m = sub ? f.m + sk * dr : f.m;
+  f.un = sub ? f.un + sk * (rs * SM - K.r * K.u) : f.un;
+  f.ut = sub ? f.ut + sk * dr * K.v : f.ut;
+  f.e = sub ? f.e + sk * (Es - EK) : f.e;
   return f;
 }
 

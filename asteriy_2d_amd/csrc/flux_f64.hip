@@ -41,6 +41,45 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const double* __res
   if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<FNT>(st, redi, stats);
 }
 
+// The exact solver with per-face newton starting points and diagnostics.
+__global__ __launch_bounds__(FNT) void guess_kernel(size_t n, const double* __restrict__ in,
+                                                    double n1, double n2, double rad,
+                                                    Phys<double> ph, const float* __restrict__ guess,
+                                                    double* __restrict__ out, ast_stats_t* stats,
+                                                    int* __restrict__ info) {
+  __shared__ uint32_t redi[(FNT / 64) * 10];
+  FaceStats st;
+  st.clear();
+  const size_t stride = (size_t)gridDim.x * FNT;
+  for (size_t i = (size_t)blockIdx.x * FNT + threadIdx.x; i < n; i += stride) {
+    V2<double> Ls = {in[i], in[n + i]};
+    V2<double> Lu = {in[2 * n + i], in[3 * n + i]};
+    V2<double> Rs = {in[4 * n + i], in[5 * n + i]};
+    V2<double> Ru = {in[6 * n + i], in[7 * n + i]};
+    V2<double> Ps, Pu;
+    const float gv = guess ? guess[i] : 0.f;
+    const bool use = gv > 0.f && gv < 3.0e38f;
+    FaceStats fs;
+    fs.clear();
+    const double t = GODUNOV(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad, fs, use, (double)gv);
+    out[i] = Ps.x;
+    out[n + i] = Ps.y;
+    out[2 * n + i] = Pu.x;
+    out[3 * n + i] = Pu.y;
+    out[4 * n + i] = t;
+    if (info) {
+      const int pat = fs.vacuum ? -1 : fs.raref ? 0 : fs.bisect ? 1 : fs.newton ? 2 : -2;
+      info[2 * i] = (int)(fs.newton_it + fs.bisect_it);
+      info[2 * i + 1] = (pat + 1) | (fs.capped ? 8 : 0);
+    }
+    fs.mlp = (use && fs.newton) ? 1 : 0;
+    st.exact += fs.exact; st.newton += fs.newton; st.newton_it += fs.newton_it;
+    st.bisect += fs.bisect; st.bisect_it += fs.bisect_it; st.raref += fs.raref;
+    st.vacuum += fs.vacuum; st.capped += fs.capped; st.mlp += fs.mlp;
+  }
+  if (stats != nullptr) block_stats_to<FNT>(st, redi, stats);
+}
+
 template <int METHOD>
 static int launch_flux(size_t n, const double* in, double n1, double n2, double rad,
                        const Phys<double>& ph, double* out, ast_stats_t* stats,
@@ -85,4 +124,20 @@ extern "C" int ast_flux_f64(int method, size_t n, const double* in, double n1, d
     case AST_EXACT: return launch_flux<AST_EXACT>(n, in, n1, n2, rad, ph, out, stats, st);
     default: return ast_fail(AST_ERR_ARG, "ast_flux_f64: unknown method %d", method);
   }
+}
+
+extern "C" int ast_flux_f64_guess(size_t n, const double* in, double n1, double n2, double rad,
+                                  double gamma, double krit, const float* guess, double* out,
+                                  ast_stats_t* stats, int* info, void* stream) {
+  using namespace ast::f64;
+  if (n == 0) return AST_OK;
+  if (!in || !out) return ast_fail(AST_ERR_ARG, "ast_flux_f64_guess: null in/out");
+  if (!(gamma > 1.0)) return ast_fail(AST_ERR_ARG, "ast_flux_f64_guess: gamma must be > 1");
+  const ast::Phys<double> ph = host_phys(gamma, krit);
+  size_t blocks = (n + FNT - 1) / FNT;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  hipLaunchKernelGGL(guess_kernel, dim3((unsigned)blocks), dim3(FNT), 0, (hipStream_t)stream, n,
+                     in, n1, n2, rad, ph, guess, out, stats, info);
+  AST_LAUNCH_CHECK("ast_flux_f64_guess launch");
+  return AST_OK;
 }

@@ -228,6 +228,71 @@ extern "C" int ast_sim_set_tile_rows(ast_sim_t* S, int krows) {
 
 extern "C" void* ast_sim_stream(ast_sim_t* S) { return S ? (void*)S->stream : nullptr; }
 
+// Tile autotuner of the FP32 step: times `reps` launches of the step kernel
+// for each candidate strip height (rows per wave) on the simulation's own
+// state and stream, keeps the fastest, and restores the time scalars (the
+// repeated launches read the current planes and only write the spare ones).
+// best_rows / best_us (may be NULL) report the choice.  The FP64 step has a
+// fixed tile; for it this only reports the step time.
+extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, int reps,
+                                int* best_rows, float* best_us) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_autotune: null sim");
+  static const int kDefault[] = {3, 4, 5, 6, 7, 8, 10, 12, 16};
+  if (!candidates || ncand <= 0) {
+    candidates = kDefault;
+    ncand = (int)(sizeof kDefault / sizeof kDefault[0]);
+  }
+  if (reps < 1) reps = 10;
+  double saved[4];
+  CK(hipStreamSynchronize(S->stream));
+  CK(hipMemcpy(saved, S->scal, sizeof saved, hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  const int from = S->cur;
+  void* sp = S->plane[2 * from];
+  void* up = S->plane[2 * from + 1];
+  void* s2 = S->plane[2 * (1 - from)];
+  void* u2 = S->plane[2 * (1 - from) + 1];
+  int best = S->krows;
+  float best_t = 1e30f;
+  const int n = S->precision == 1 ? ncand : 1;
+  for (int c = 0; c < n; c++) {
+    const int kr = S->precision == 1 ? candidates[c] : 0;
+    if (S->precision == 1 && (kr < 1 || kr > 1024)) continue;
+    for (int r = -2; r < reps; r++) {  // two untimed launches first
+      if (r == 0) CK(hipEventRecord(e0, S->stream));
+      int rc = S->precision == 1
+                   ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
+                                       (float*)u2, S->scal, S->scal + 1, S->method, S->mlp,
+                                       nullptr, kr, S->stream)
+                   : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
+                                  (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
+                                  nullptr, S->stream);
+      if (rc != AST_OK) return rc;
+    }
+    CK(hipEventRecord(e1, S->stream));
+    CK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    const float us = 1000.f * ms / reps;
+    if (us < best_t) {
+      best_t = us;
+      best = kr;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CK(hipMemcpy(S->scal, saved, sizeof saved, hipMemcpyHostToDevice));
+  if (S->precision == 1 && best != S->krows) {
+    S->krows = best;
+    drop_graph(S);
+  }
+  if (best_rows) *best_rows = S->precision == 1 ? best : 0;
+  if (best_us) *best_us = best_t;
+  return AST_OK;
+}
+
 static int build_graph(ast_sim_t* S) {
   drop_graph(S);
   CK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));

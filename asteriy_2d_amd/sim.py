@@ -29,6 +29,8 @@ SIM_SIGNATURES = {
     "ast_sim_set_mlp": (_c_int, [_vp, _vp]),
     "ast_sim_set_tile_rows": (_c_int, [_vp, _c_int]),
     "ast_sim_stream": (_vp, [_vp]),
+    "ast_sim_autotune": (_c_int, [_vp, ctypes.POINTER(_c_int), _c_int, _c_int,
+                                  ctypes.POINTER(_c_int), ctypes.POINTER(ctypes.c_float)]),
     "ast_sim_run": (_c_int, [_vp, _c_int, _c_int]),
     "ast_sim_sync": (_c_int, [_vp]),
     "ast_sim_write_b31": (_c_int, [_vp, ctypes.c_char_p]),
@@ -118,6 +120,18 @@ class Simulation:
 
     def set_tile_rows(self, krows: int):
         native.call("ast_sim_set_tile_rows", self.h, int(krows))
+
+    def autotune(self, candidates=None, reps: int = 10):
+        """Picks the fastest FP32 strip height for this grid (csrc/sim.cpp);
+        returns (rows, microseconds per step)."""
+        arr, n = None, 0
+        if candidates:
+            n = len(candidates)
+            arr = (_c_int * n)(*candidates)
+        r, t = _c_int(), ctypes.c_float()
+        native.call("ast_sim_autotune", self.h, arr, n, int(reps), ctypes.byref(r),
+                    ctypes.byref(t))
+        return r.value, t.value
 
     @property
     def stream_ptr(self) -> int:

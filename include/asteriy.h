@@ -166,7 +166,7 @@ int ast_funk_time(double* T, double* T_do, double* TT, int* i, void* stream);
  * diagnostics: out[0] = sum rho*w*dV, out[1] = sum E*w*dV with
  * E = p/(gamma-1) + rho*|u|^2/2 and w = 1 (planar) or y (axisymmetric).
  * `fp32` selects float (1) or double (0) state planes.  out is a device
- * pointer to 2 doubles. */
+ * buffer of ast_conservation_work_doubles() doubles (results first). */
 int ast_conservation_sums(const ast_grid_t* g, const void* s, const void* u,
                           int fp32, double* out, void* stream);
 
@@ -174,49 +174,142 @@ int ast_conservation_sums(const ast_grid_t* g, const void* s, const void* u,
 int ast_convert_f64_to_f32(size_t n, const double* src, float* dst, void* stream);
 int ast_convert_f32_to_f64(size_t n, const float* src, double* dst, void* stream);
 
+/* Same as ast_flux_f32, with per-face solver diagnostics for the exact
+ * solver: info (may be NULL) receives 2 ints per face: the newton/root
+ * iterations, and flags = (pattern + 1) | 8*capped | 16*fp64_island with
+ * pattern -1 vacuum, 0 two rarefactions, 1 shock+rarefaction, 2 two shocks. */
+int ast_flux_f32_info(int method, size_t n, const float* in, float n1, float n2,
+                      float rad, float gamma, float krit, float* out,
+                      ast_stats_t* stats, int* info, void* stream);
+
+/* FP64 exact solver (Godunov_Solver_Alexashov, kernel.cu:128) over a batch
+ * with per-face newton starting points: guess (device, n floats, may be
+ * NULL) holds p* guesses (<= 0 or NaN = the reference's PVRS start).  The
+ * stopping rule is the reference's (|dp/pee| < 1e-9, pee = PVRS estimate),
+ * so the guess changes the iteration count only.  info as above (FP64
+ * faces never set the island flag). */
+int ast_flux_f64_guess(size_t n, const double* in, double n1, double n2,
+                       double rad, double gamma, double krit, const float* guess,
+                       double* out, ast_stats_t* stats, int* info, void* stream);
+
 /* ---- learned p* initial guess ------------------------------------------
  * A 6 -> 32 -> 32 -> 32 -> 1 tanh MLP maps the normalised primitives of a
- * two-shock face to log(p_star / p_ref); it only replaces the starting point of
- * newton (kernel.cu:641-642), whose stopping rule keeps the PVRS guess as
- * its scale.  Parameters are FP32 masters (trained) and BF16 copies
- * (inference on MFMA).  ast_mlp_t lives in device memory and is laid out
- * as AST_MLP_NPARAM floats followed by the BF16 copy; create it with
- * ast_mlp_alloc-free helpers below on caller-owned memory. */
+ * two-shock face to log(p_star / p_ref); it only replaces the starting point
+ * of newton (kernel.cu:641-642), whose stopping rule keeps the PVRS guess as
+ * its scale.  An ast_mlp_t is a block of ast_mlp_bytes() bytes of device
+ * memory (caller-allocated): AST_MLP_NPARAM FP32 master parameters
+ * (W1[32][6] b1[32] W2[32][32] b2 W3[32][32] b3 w4[32] b4) followed by the
+ * BF16 MFMA operands derived from them. */
 #define AST_MLP_IN 6
 #define AST_MLP_HID 32
 #define AST_MLP_NPARAM (AST_MLP_IN * AST_MLP_HID + AST_MLP_HID + \
                         2 * (AST_MLP_HID * AST_MLP_HID + AST_MLP_HID) + \
                         AST_MLP_HID + 1)
-/* Bytes of device memory an ast_mlp_t occupies. */
 size_t ast_mlp_bytes(void);
+/* Floats of workspace ast_mlp_loss_grad needs (per-workgroup partials). */
+size_t ast_mlp_work_floats(void);
 
-/* Normalised features of n faces given as the 8-plane SoA of
- * ast_flux_f64 (normal (n1,n2)): feats is 6 planes of n floats, target
- * (may be NULL) receives log(p_star / p_ref) from the FP64 exact solve, and
- * mask (may be NULL) is 1 for two-shock faces (the only ones the guess
- * serves), else 0. */
+/* The solver's face log: normalised features of n faces given as the
+ * 8-plane SoA of ast_flux_f64 (normal (n1,n2)): feats is 6 planes of n
+ * floats; target (may be NULL) receives log(p_star / p_ref) from the FP64
+ * exact solve at the reference tolerance; mask (may be NULL) is 1 for
+ * two-shock faces (the only ones the guess serves), else 0. */
 int ast_mlp_features(size_t n, const double* in, double n1, double n2,
                      double gamma, float* feats, float* target,
                      unsigned char* mask, void* stream);
-/* Forward pass of n feature rows; pred receives n floats.  precision 0:
- * FP32 master weights on the vector ALU; 1: BF16 weights on MFMA with
- * FP32 accumulation. */
+/* Forward pass of n feature rows into pred (n floats).  precision 0: FP32
+ * masters on the vector ALU; 1: BF16 operands on MFMA, FP32 accumulate. */
 int ast_mlp_forward(const ast_mlp_t* mlp, size_t n, const float* feats,
                     float* pred, int precision, void* stream);
-/* Mean-squared-error loss and its gradient w.r.t. the FP32 parameters
- * over n rows (mask may be NULL): grad receives AST_MLP_NPARAM floats,
- * loss one float. */
+/* Mean-squared-error loss over the rows with mask != 0 (mask may be NULL)
+ * and its gradient w.r.t. the FP32 masters: grad receives AST_MLP_NPARAM
+ * floats, loss one float (device pointers); work holds
+ * ast_mlp_work_floats() floats.  Deterministic. */
 int ast_mlp_loss_grad(const ast_mlp_t* mlp, size_t n, const float* feats,
                       const float* target, const unsigned char* mask,
-                      float* grad, float* loss, void* stream);
-/* One Adam step on the FP32 masters from grad (state: 2*NPARAM floats,
- * zero-initialised by the caller), then refreshes the BF16 copy. */
+                      float* grad, float* loss, float* work, void* stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8; state: 2*NPARAM floats,
+ * zero-initialised by the caller, step >= 1) on the masters, then the BF16
+ * operands are rebuilt. */
 int ast_mlp_adam(ast_mlp_t* mlp, const float* grad, float* state, int step,
                  float lr, void* stream);
-/* Copies NPARAM host floats into the masters and refreshes the BF16 copy
- * (synchronous). */
+/* Rebuilds the BF16 operands from the masters. */
+int ast_mlp_refresh(ast_mlp_t* mlp, void* stream);
+/* Copies NPARAM host floats into / out of the masters (synchronous). */
 int ast_mlp_set_params(ast_mlp_t* mlp, const float* host_params, void* stream);
 int ast_mlp_get_params(const ast_mlp_t* mlp, float* host_params, void* stream);
+
+/* ---- the host runtime: the reference's time loop ------------------------
+ * Replaces main()'s protocol (kernel.cu:3868-3906: Kernel_TVD, funk_time,
+ * ping-pong by swapping arguments, a device sync after every launch) with a
+ * native object: it owns the two pairs of state planes and the device time
+ * scalars (T, T_do, TT, i; initialised to 1e7, 1e-9, 0, 0 as in
+ * kernel.cu:3321-3324) and replays two steps at a time from a captured HIP
+ * graph on its own stream.  precision 0 = FP64 (ast_step_f64), 1 = FP32
+ * (ast_step_f32).  Host state crosses in the reference layout (N*M double
+ * pairs s = {rho,p}, u = {vx,vy}). */
+typedef struct ast_sim ast_sim_t;
+int ast_sim_create(const ast_grid_t* g, int precision, int method, ast_sim_t** out);
+int ast_sim_destroy(ast_sim_t* sim);
+int ast_sim_set_state(ast_sim_t* sim, const double* s, const double* u);
+int ast_sim_get_state(ast_sim_t* sim, double* s, double* u);
+/* Device planes holding the current state (FP64 or FP32 pairs). */
+int ast_sim_device_state(ast_sim_t* sim, void** s, void** u);
+int ast_sim_set_time(ast_sim_t* sim, double T, double T_do, double TT, int i);
+int ast_sim_get_time(ast_sim_t* sim, double* T, double* T_do, double* TT, int* i);
+/* Exact-solver counters accumulated by the simulation (reset on request). */
+int ast_sim_get_stats(ast_sim_t* sim, ast_stats_t* out, int reset);
+/* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step. */
+int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
+/* Strip height (rows per wave) of the FP32 step. */
+int ast_sim_set_tile_rows(ast_sim_t* sim, int rows);
+/* Times the FP32 step for each candidate strip height (NULL = a default
+ * list), keeps the fastest; reports it (may be NULL).  State and time are
+ * left unchanged. */
+int ast_sim_autotune(ast_sim_t* sim, const int* candidates, int ncand, int reps,
+                     int* best_rows, float* best_us);
+/* The simulation's stream (hipStream_t) for event timing. */
+void* ast_sim_stream(ast_sim_t* sim);
+/* Advances `steps` steps asynchronously (use_graph: replay the captured
+ * two-step graph). */
+int ast_sim_run(ast_sim_t* sim, int steps, int use_graph);
+int ast_sim_sync(ast_sim_t* sim);
+/* The reference's binary state file (kernel.cu:3414-3436 read,
+ * 4111-4148 write): per cell in k order, 6 doubles x, y, rho, p, vx, vy. */
+int ast_sim_write_b31(ast_sim_t* sim, const char* path);
+int ast_sim_read_b31(ast_sim_t* sim, const char* path);
+/* Checkpoint with the time state (T, T_do, TT, i) the reference does not
+ * save: a resumed run continues bit for bit. */
+int ast_sim_save(ast_sim_t* sim, const char* path);
+int ast_sim_load(ast_sim_t* sim, const char* path);
+
+/* ---- batched parameter sweep ------------------------------------------
+ * Many independent FP32 instances (uneven grids, own physics, own dt) in
+ * one persistent launch per step: a fixed grid of workgroups walks the
+ * strip table of all instances; a second kernel advances every instance's
+ * time (funk_time per instance); both are captured in one HIP graph.
+ * Instances share the method.  Tile heights are chosen per grid shape by
+ * ast_batch_autotune. */
+typedef struct ast_batch ast_batch_t;
+int ast_batch_create(int count, const ast_grid_t* grids, int method, ast_batch_t** out);
+int ast_batch_destroy(ast_batch_t* b);
+int ast_batch_set_state(ast_batch_t* b, int inst, const double* s, const double* u);
+int ast_batch_get_state(ast_batch_t* b, int inst, double* s, double* u);
+int ast_batch_get_time(ast_batch_t* b, int inst, double* T, double* T_do, double* TT, int* i);
+/* Per-shape strip heights: times candidate heights on the batch itself. */
+int ast_batch_autotune(ast_batch_t* b, int reps);
+int ast_batch_run(ast_batch_t* b, int steps, int use_graph);
+int ast_batch_sync(ast_batch_t* b);
+void* ast_batch_stream(ast_batch_t* b);
+/* Total cells of all instances (cell-updates per step). */
+long long ast_batch_cells(const ast_batch_t* b);
+
+/* ---- tile configuration of the stand-alone FP32 step entry point -------- */
+int ast_set_tile_rows(int rows);
+int ast_get_tile_rows(void);
+/* Doubles of device memory ast_conservation_sums needs at `out` (the two
+ * results followed by its partial sums). */
+size_t ast_conservation_work_doubles(void);
 
 #ifdef __cplusplus
 }

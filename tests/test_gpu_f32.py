@@ -65,8 +65,12 @@ def test_flux_f32_vs_f64(dev, method):
     tol = {G.HLL: (1e-7, 1e-5, 1e-4), G.HLLC: (1e-7, 3e-4, 2e-2),
            G.EXACT: (1e-7, 3e-4, 0.2)}[method]
     assert q[0] < tol[0] and q[1] < tol[1] and q[2] < tol[2]
+    # dt: the exact solver's largest wave speed depends on the wave pattern,
+    # and faces on a pattern boundary may classify differently in FP32
     dt_rel = np.abs(o32[4] - o64[4]) / o64[4]
-    assert np.quantile(dt_rel, 0.999) < 1e-4
+    assert np.quantile(dt_rel, 0.99) < 1e-4
+    if method != G.EXACT:
+        assert np.quantile(dt_rel, 0.999) < 1e-4
     if method == G.EXACT:
         d = native.stats_from_tensor(st)
         assert d["capped"] == 0, d
