@@ -20,6 +20,7 @@
 #include <math.h>
 
 #include "fast_flux.hpp"
+#include "mlp.hpp"
 #include "ref_flux.hpp"
 
 namespace ast {
@@ -27,7 +28,7 @@ namespace ast {
 struct ExactCtx {
   float g, g1, g2, gp, gm, igm, ig1, c2g1, g1g2, c2g2;
   double gd;
-  const float* mlp;  // learned-guess parameters in device memory, or null
+  const unsigned char* mlp;  // learned-guess block (ast_mlp_t) in device memory, or null
 };
 
 static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
@@ -45,7 +46,7 @@ static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
   e.g1g2 = (float)(g1 / g2);
   e.c2g2 = (float)(2.0 / g2);
   e.gd = gamma;
-  e.mlp = (const float*)mlp;
+  e.mlp = (const unsigned char*)mlp;
   return e;
 }
 
@@ -53,6 +54,9 @@ static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
 AST_DEV float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
 
 constexpr float kNewtonTol32 = 1e-6f;
+// a two-shock face is served by the learned guess when the PVRS start is
+// poor: PVRS estimate above kStrongShock * pII (strong shocks)
+constexpr float kStrongShock = 3.f;
 constexpr float kVacuumIsland = 0.05f;
 constexpr float kRarefIsland = 1e-4f;
 
@@ -142,6 +146,10 @@ AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R
 
 // The exact face flux in the face frame (u normal, v tangential); UU is the
 // largest |wave speed| used for dt.  `guess` (finite, > 0) seeds newton.
+// With e.mlp set the caller guarantees a full wave (EXEC all ones): if any
+// lane of the wave holds a strong two-shock face, the wave evaluates the
+// learned guess for all its lanes on MFMA (a wave-uniform branch) and those
+// lanes start newton from it.
 AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
                         FaceStats& st, float guess = -1.f, int* info = nullptr) {
   const float g = e.g, g1 = e.g1, g2 = e.g2, gm = e.gm;
@@ -158,6 +166,17 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   const float Uvk = -e.c2g1 * (cII + cI);
   const float Udf = enI - enII;
   st.exact += 1;
+  if (e.mlp != nullptr) {
+    const float zI = rI * cI, zII = rII * cII;
+    const float pvrs = (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII);
+    const bool want = (Udf > Uud) && (pvrs > kStrongShock * pII);
+    if (__builtin_amdgcn_ballot_w64(want) != 0) {  // wave-uniform
+      float f[mlp::IN];
+      const float lp = mlp::features<float>(rI, pI, enI, rII, pII, enII, g, f);
+      const float y = mlp::forward_wave_bf16(e.mlp, f);
+      if (want) guess = __expf(y + lp);
+    }
+  }
   int pat;
   float p = 0.f;
   int it = 0;

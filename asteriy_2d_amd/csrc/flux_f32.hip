@@ -19,8 +19,17 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
   __shared__ uint32_t redi[(FNT / 64) * 10];
   FaceStats st;
   st.clear();
+  // whole 64-face tiles per wave: every lane runs every iteration (EXEC full,
+  // as the wave-level learned guess needs); lanes past n compute on a clamped
+  // face and do not store
   const size_t stride = (size_t)gridDim.x * FNT;
-  for (size_t i = (size_t)blockIdx.x * FNT + threadIdx.x; i < n; i += stride) {
+  const size_t nt = (n + 63) / 64 * 64;
+  const int lane = threadIdx.x & 63;
+  const size_t base0 = (size_t)blockIdx.x * FNT + 64 * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (size_t base = base0; base < nt; base += stride) {  // wave-uniform
+    const size_t ii = base + lane;
+    const bool valid = ii < n;
+    const size_t i = valid ? ii : n - 1;
     const float rl = in[i], pl = in[n + i], vxl = in[2 * n + i], vyl = in[3 * n + i];
     const float rr = in[4 * n + i], pr = in[5 * n + i], vxr = in[6 * n + i], vyr = in[7 * n + i];
     // face frame: normal (n1, n2), tangent (-n2, n1)
@@ -33,16 +42,18 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
     else {
       int inf[2];
       f = exact_f32(L, R, ex, UU, st, -1.f, info ? inf : nullptr);
-      if (info) {
+      if (info && valid) {
         info[2 * i] = inf[0];
         info[2 * i + 1] = inf[1];
       }
     }
-    out[i] = f.m;
-    out[n + i] = f.e;
-    out[2 * n + i] = f.un * n1 - f.ut * n2;
-    out[3 * n + i] = f.un * n2 + f.ut * n1;
-    out[4 * n + i] = krad * frcp(UU);
+    if (valid) {
+      out[i] = f.m;
+      out[n + i] = f.e;
+      out[2 * n + i] = f.un * n1 - f.ut * n2;
+      out[3 * n + i] = f.un * n2 + f.ut * n1;
+      out[4 * n + i] = krad * frcp(UU);
+    }
   }
   if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<FNT>(st, redi, stats);
 }
@@ -62,13 +73,13 @@ static int launch(size_t n, const float* in, float n1, float n2, float krad, con
 }  // namespace ast
 
 extern "C" int ast_flux_f32_info(int method, size_t n, const float* in, float n1, float n2,
-                                 float rad, float gamma, float krit, float* out,
-                                 ast_stats_t* stats, int* info, void* stream) {
+                                 float rad, float gamma, float krit, const ast_mlp_t* mlp,
+                                 float* out, ast_stats_t* stats, int* info, void* stream) {
   using namespace ast::f32;
   if (n == 0) return AST_OK;
   if (!in || !out) return ast_fail(AST_ERR_ARG, "ast_flux_f32: null in/out");
   if (!(gamma > 1.0f)) return ast_fail(AST_ERR_ARG, "ast_flux_f32: gamma must be > 1");
-  const ast::ExactCtx ex = ast::make_exact_ctx((double)gamma, nullptr);
+  const ast::ExactCtx ex = ast::make_exact_ctx((double)gamma, mlp);
   const float krad = krit * rad;
   hipStream_t st = (hipStream_t)stream;
   switch (method) {
@@ -82,5 +93,6 @@ extern "C" int ast_flux_f32_info(int method, size_t n, const float* in, float n1
 extern "C" int ast_flux_f32(int method, size_t n, const float* in, float n1, float n2, float rad,
                             float gamma, float krit, float* out, ast_stats_t* stats,
                             void* stream) {
-  return ast_flux_f32_info(method, n, in, n1, n2, rad, gamma, krit, out, stats, nullptr, stream);
+  return ast_flux_f32_info(method, n, in, n1, n2, rad, gamma, krit, nullptr, out, stats, nullptr,
+                           stream);
 }

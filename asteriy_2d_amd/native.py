@@ -36,7 +36,7 @@ SIGNATURES = {
     "ast_convert_f64_to_f32": (_c_int, [_c_size, _vp, _vp, _vp]),
     "ast_convert_f32_to_f64": (_c_int, [_c_size, _vp, _vp, _vp]),
     "ast_flux_f32_info": (_c_int, [_c_int, _c_size, _vp, _c_float, _c_float, _c_float, _c_float,
-                                   _c_float, _vp, _vp, _vp, _vp]),
+                                   _c_float, _vp, _vp, _vp, _vp, _vp]),
     "ast_set_tile_rows": (_c_int, [_c_int]),
     "ast_get_tile_rows": (_c_int, []),
 }

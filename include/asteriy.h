@@ -151,7 +151,7 @@ int ast_step_f64(const ast_grid_t* g, const double* s, const double* u,
                  double* s2, double* u2, double* T, const double* T_do,
                  double* TT, int method, ast_stats_t* stats, void* stream);
 
-typedef struct ast_mlp ast_mlp_t;   /* see the learned-guess section below */
+typedef struct ast_mlp ast_mlp_t;   /* the learned guess, see below */
 
 int ast_step_f32(const ast_grid_t* g, const float* s, const float* u,
                  float* s2, float* u2, double* T, const double* T_do,
@@ -174,13 +174,16 @@ int ast_conservation_sums(const ast_grid_t* g, const void* s, const void* u,
 int ast_convert_f64_to_f32(size_t n, const double* src, float* dst, void* stream);
 int ast_convert_f32_to_f64(size_t n, const float* src, double* dst, void* stream);
 
-/* Same as ast_flux_f32, with per-face solver diagnostics for the exact
- * solver: info (may be NULL) receives 2 ints per face: the newton/root
- * iterations, and flags = (pattern + 1) | 8*capped | 16*fp64_island with
- * pattern -1 vacuum, 0 two rarefactions, 1 shock+rarefaction, 2 two shocks. */
+/* Same as ast_flux_f32, with the learned newton start (mlp: device
+ * ast_mlp_t, NULL = off; used by strong two-shock faces) and per-face
+ * solver diagnostics for the exact solver: info (may be NULL) receives 2
+ * ints per face: the newton/root iterations, and flags = (pattern + 1) |
+ * 8*capped | 16*fp64_island with pattern -1 vacuum, 0 two rarefactions,
+ * 1 shock+rarefaction, 2 two shocks. */
+typedef struct ast_mlp ast_mlp_t;
 int ast_flux_f32_info(int method, size_t n, const float* in, float n1, float n2,
-                      float rad, float gamma, float krit, float* out,
-                      ast_stats_t* stats, int* info, void* stream);
+                      float rad, float gamma, float krit, const ast_mlp_t* mlp,
+                      float* out, ast_stats_t* stats, int* info, void* stream);
 
 /* FP64 exact solver (Godunov_Solver_Alexashov, kernel.cu:128) over a batch
  * with per-face newton starting points: guess (device, n floats, may be
