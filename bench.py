@@ -1,22 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit (z,r) finite-volume step on one MI355X.
 
-Workload: the stellar-wind / counter-flow interaction on a 2048x1024 (z,r)
-grid, z in [-400,400], r in [0,400], gamma 5/3, HLLC + minmod, CFL 0.3
+Headline workload: the stellar-wind / counter-flow interaction on a 2048x1024
+(z,r) grid, z in [-400,400], r in [0,400], gamma 5/3, HLLC + minmod, CFL 0.3
 (asteriy_2d_amd/problems.py).  A "step" is one pass of the step kernel over
-the grid followed by the time advance (the reference's Kernel_TVD + funk_time
-pair), replayed from a captured HIP graph with the state resident in HBM.
+the grid followed by the time advance (the reference's Kernel_TVD +
+funk_time), replayed from a captured HIP graph with the state resident in HBM.
 
-Headline `value`: Gcell-updates/s of the FP32 fast step over the timed steps.
-The same run also measures the plain FP64 step (the bit-faithful restatement
-of the reference step) on the same workload and the L1 density/pressure error
-of the FP32 run against it, and the exact-solver FP32 step.
+`value`: Gcell-updates/s of the FP32 fast step over the K timed steps.  The
+same line carries the plain FP64 step (the bit-faithful restatement of the
+reference step) on the same workload and the FP32 run's L1 density/pressure
+error against it, together with the FP64 run's own sensitivity floor (FP64
+started from the FP32-rounded initial state); the exact-solver step (FP32
+with FP64 islands vs FP64) at 2048x1024; the exact solver with the learned
+newton start at 4096x2048; the 64-instance batched sweep; the roofline of
+the headline kernel; and the CPU baseline.
 
 Roofline: algorithmic bytes of the step kernel (FP32: 16 B read + 16 B
 written per cell-update) over its average launch duration, timed with HIP
-events on the stream the kernel runs on; peak 8 TB/s HBM.
-CPU baseline: the oracle (line-faithful C restatement of the reference step,
-FP64, OpenMP) on the host cores, on a bounded sample of full-grid steps.
+events on the stream the kernel runs on; peak 8 TB/s HBM.  CPU baseline: the
+oracle (line-faithful C restatement of the reference step, FP64, OpenMP) on
+the host cores, on a bounded sample of full-grid steps.
+Progress goes to stderr; stdout is the one JSON line.
 """
 from __future__ import annotations
 
@@ -33,28 +38,31 @@ HBM_PEAK_GBS = 8000.0
 N_GRID, M_GRID = 2048, 1024
 
 
-def _timed(sim, steps: int, stream):
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _timed(runner, steps: int, stream):
     import torch
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    runner.sync()
     torch.cuda.synchronize()
-    sim.sync()
     e0.record(stream)
-    sim.run(steps, graph=True)
+    runner.run(steps, graph=True)
     e1.record(stream)
-    sim.sync()
+    runner.sync()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1)
 
 
-def _kernel_ms(grid, precision, method, stream, reps=50):
+def _kernel_ms(grid, precision, method, stream, krows=None, reps=50):
     """Average duration of the step kernel alone: `reps` back-to-back launches
     on `stream` between two HIP events (the queue stays full, so the launch
     gaps are hidden)."""
-    import numpy as np
     import torch
 
-    from asteriy_2d_amd import problems, solver
+    from asteriy_2d_amd import native, problems, solver
     dt = torch.float32 if precision == "f32" else torch.float64
     s, u = problems.wind_state(grid)
     dev = torch.device("cuda:0")
@@ -64,18 +72,21 @@ def _kernel_ms(grid, precision, method, stream, reps=50):
     T = torch.tensor([1e7], dtype=torch.float64, device=dev)
     Td = torch.tensor([0.05], dtype=torch.float64, device=dev)
     TT = torch.zeros(1, dtype=torch.float64, device=dev)
+    L = native.load()
+    old = L.ast_get_tile_rows()
+    if krows:
+        L.ast_set_tile_rows(int(krows))
     torch.cuda.synchronize()
-    with torch.cuda.stream(stream):
-        for _ in range(3):
-            solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream)
-        e1.record(stream)
+    for _ in range(3):
+        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream)
+    e1.record(stream)
     torch.cuda.synchronize()
-    del np
+    L.ast_set_tile_rows(old)
     return e0.elapsed_time(e1) / reps
 
 
@@ -84,23 +95,46 @@ def _l1(a, b):
     return float(np.abs(a - b).sum() / np.abs(b).sum())
 
 
+def _linf(a, b):
+    import numpy as np
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _errors(x, ref):
+    return {"l1_rho": _l1(x[0][..., 0], ref[0][..., 0]), "l1_p": _l1(x[0][..., 1], ref[0][..., 1]),
+            "linf_rho": _linf(x[0][..., 0], ref[0][..., 0]),
+            "linf_p": _linf(x[0][..., 1], ref[0][..., 1])}
+
+
+def _run_sim(grid, method, precision, s, u, steps, mlp=None, autotune=False):
+    from asteriy_2d_amd.sim import Simulation
+    sim = Simulation(grid, method, precision, s, u)
+    if mlp is not None:
+        sim.set_mlp(mlp)
+    rows = None
+    if autotune and precision == "f32":
+        rows, _ = sim.autotune()
+    ms = _timed(sim, steps, sim.torch_stream())
+    st = sim.state()
+    t = sim.time()
+    stats = sim.stats()
+    sim.close()
+    return ms, st, t, stats, rows
+
+
 def _cpu_baseline(grid, s, u, method, budget_s: float):
     """Oracle steps on the full grid until ~budget_s of CPU time."""
-    import numpy as np
-
     from oracle import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     O.build()
-    T_do = 0.05
     steps = 0
     t0 = time.perf_counter()
     while True:
-        O.step(grid, s, u, 1e7, T_do, method, nthreads=threads)
+        O.step(grid, s, u, 1e7, 0.05, method, nthreads=threads)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 50:
+        if el >= budget_s or steps >= 500:
             break
-    del np
     return {"value": round(grid.cells * steps / el / 1e9, 6), "unit": "Gcell-updates/s",
             "cores": threads, "kind": "port",
             "sample": f"{steps} full-grid FP64 HLLC steps of the {grid.N}x{grid.M} wind workload "
@@ -126,13 +160,18 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
+    ap.add_argument("--profile", choices=["f32", "f64"], default=None,
+                    help="only launch the headline step kernel --steps times (eager, for rocprofv3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB buffer --steps times (counter calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
         return 2
 
+    import numpy as np
     import torch
 
     from asteriy_2d_amd import EXACT, HLLC, native, problems
@@ -146,69 +185,132 @@ def main(argv=None):
     g = p.grid
     cells = g.cells
     K, W = a.steps, a.warmup
+    if a.calibrate_copy:
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(K):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {K} x {src.numel() * 4} bytes")
+        return 0
+    if a.profile:
+        rows = 8
+        ms = _kernel_ms(g, a.profile, HLLC, torch.cuda.current_stream(), krows=rows, reps=K)
+        byt = (32.0 if a.profile == "f32" else 64.0) * cells
+        log(f"step_{a.profile} HLLC {N_GRID}x{M_GRID}: {ms * 1e3:.2f} us/launch, "
+            f"{byt / (ms * 1e-3) / 1e9:.1f} GB/s algorithmic")
+        return 0
 
     # -- FP32 fast step (headline) ------------------------------------------
     sim32 = Simulation(g, HLLC, "f32", p.s, p.u)
+    rows, _ = sim32.autotune()
     st32 = sim32.torch_stream()
     sim32.run(W).sync()
     sim32.set_state(p.s, p.u)
     sim32.set_time()
     ms32 = _timed(sim32, K, st32)
-    s32, u32 = sim32.state()
+    r32 = sim32.state()
     t32 = sim32.time()
-    kms32 = _kernel_ms(g, "f32", HLLC, st32)
+    kms32 = _kernel_ms(g, "f32", HLLC, st32, krows=rows)
+    sim32.close()
+    log(f"FP32 HLLC: {cells * K / ms32 / 1e6:.1f} Gcell/s, kernel {kms32 * 1e3:.1f} us, rows {rows}")
 
-    # -- plain FP64 step on the same workload --------------------------------
-    sim64 = Simulation(g, HLLC, "f64", p.s, p.u)
-    st64 = sim64.torch_stream()
-    sim64.run(W).sync()
-    sim64.set_state(p.s, p.u)
-    sim64.set_time()
-    ms64 = _timed(sim64, K, st64)
-    s64, u64 = sim64.state()
-    t64 = sim64.time()
-    kms64 = _kernel_ms(g, "f64", HLLC, st64, reps=20)
-
+    # -- plain FP64 step, same workload, and its sensitivity floor ------------
+    ms64, r64, t64, _, _ = _run_sim(g, HLLC, "f64", p.s, p.u, K)
+    kms64 = _kernel_ms(g, "f64", HLLC, torch.cuda.current_stream(), reps=20)
+    log(f"FP64 HLLC: {cells * K / ms64 / 1e6:.1f} Gcell/s")
     value = cells * K / (ms32 * 1e-3) / 1e9
     v64 = cells * K / (ms64 * 1e-3) / 1e9
-    alg_bytes = 32.0 * cells
-    achieved = alg_bytes / (kms32 * 1e-3) / 1e9
-    traffic = _traffic("step_f32_hllc_2048x1024")
-
+    achieved = 32.0 * cells / (kms32 * 1e-3) / 1e9
     extra = {
         "fp64_gcells": round(v64, 3),
         "fp32_vs_fp64": round(value / v64, 3),
-        "l1_rho_vs_fp64": _l1(s32[..., 0], s64[..., 0]),
-        "l1_p_vs_fp64": _l1(s32[..., 1], s64[..., 1]),
-        "sim_time": [t32["TT"], t64["TT"]],
-        "kernel_us": {"step_f32_hllc": round(kms32 * 1e3, 2), "step_f64_hllc": round(kms64 * 1e3, 2)},
-        "fp64_roofline_frac": round(64.0 * cells / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "error_vs_fp64": _errors(r32, r64),
+        "sim_time": {"fp32": t32["TT"], "fp64": t64["TT"]},
+        "tile_rows": rows,
+        "kernel_us": {"step_f32_hllc": round(kms32 * 1e3, 2),
+                      "step_f64_hllc": round(kms64 * 1e3, 2)},
+        "fp64_hbm_frac": round(64.0 * cells / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
-    sim32.close()
-    sim64.close()
+    del r32
+    if not a.quick:
+        sr = p.s.astype(np.float32).astype(np.float64)
+        ur = p.u.astype(np.float32).astype(np.float64)
+        _, rfl, _, _, _ = _run_sim(g, HLLC, "f64", sr, ur, K)
+        extra["fp64_floor_vs_fp64"] = _errors(rfl, r64)
+        del rfl
+        # error at a short horizon as well (the interface amplifies
+        # structured perturbations; see fp64_floor_vs_fp64)
+        Ks = min(K, 300)
+        _, a32, _, _, _ = _run_sim(g, HLLC, "f32", p.s, p.u, Ks)
+        _, a64, _, _, _ = _run_sim(g, HLLC, "f64", p.s, p.u, Ks)
+        extra["error_vs_fp64_300_steps"] = _errors(a32, a64)
+    del r64
 
-    # -- exact solver: FP32 (+FP64 islands) vs plain FP64, fewer steps -------
-    if not a.no_exact:
-        Ke = max(10, min(K, 200))
+    if not a.quick:
+        # -- exact solver at 2048x1024: FP32 (+FP64 islands) vs FP64 ----------
+        Ke = 200
         pe = problems.wind(N_GRID, M_GRID, method=EXACT, steps=Ke)
-        se32 = Simulation(g, EXACT, "f32", pe.s, pe.u)
-        mse32 = _timed(se32, Ke, se32.torch_stream())
-        e32 = se32.state()
-        stats = se32.stats()
-        se64 = Simulation(g, EXACT, "f64", pe.s, pe.u)
-        mse64 = _timed(se64, Ke, se64.torch_stream())
-        e64 = se64.state()
-        extra["exact"] = {
+        me32, e32, _, sx, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, autotune=True)
+        me64, e64, _, s64x, _ = _run_sim(g, EXACT, "f64", pe.s, pe.u, Ke)
+        extra["exact_2048x1024"] = {
             "steps": Ke,
-            "fp32_gcells": round(cells * Ke / (mse32 * 1e-3) / 1e9, 3),
-            "fp64_gcells": round(cells * Ke / (mse64 * 1e-3) / 1e9, 3),
-            "speedup": round(mse64 / mse32, 3),
-            "l1_rho_vs_fp64": _l1(e32[0][..., 0], e64[0][..., 0]),
-            "newton_iters_per_face": round(stats["newton_iters"] / max(1, stats["newton_faces"]), 3),
-            "fp64_island_faces": stats["fp64_faces"],
+            "fp32_gcells": round(cells * Ke / me32 / 1e6, 3),
+            "fp64_gcells": round(cells * Ke / me64 / 1e6, 3),
+            "speedup": round(me64 / me32, 3),
+            "error_vs_fp64": _errors(e32, e64),
+            "newton_iters_per_face_fp32": round(sx["newton_iters"] / max(1, sx["newton_faces"]), 3),
+            "newton_iters_per_face_fp64": round(s64x["newton_iters"] / max(1, s64x["newton_faces"]), 3),
+            "fp64_island_faces": sx["fp64_faces"],
         }
-        se32.close()
-        se64.close()
+        log(f"exact 2048: {extra['exact_2048x1024']}")
+        del e32, e64
+
+        # -- exact + learned newton start at 4096x2048 ------------------------
+        from asteriy_2d_amd import learned
+        mdl = learned.LearnedGuess.load()
+        pc = problems.wind(4096, 2048, method=EXACT)
+        gc = pc.grid
+        Kc = 100
+        mc_m, c_m, _, sc_m, _ = _run_sim(gc, EXACT, "f32", pc.s, pc.u, Kc, mlp=mdl, autotune=True)
+        mc_0, _, _, sc_0, _ = _run_sim(gc, EXACT, "f32", pc.s, pc.u, Kc, autotune=True)
+        mc_64, c_64, _, _, _ = _run_sim(gc, EXACT, "f64", pc.s, pc.u, Kc)
+        extra["exact_learned_4096x2048"] = {
+            "steps": Kc,
+            "fp32_learned_gcells": round(gc.cells * Kc / mc_m / 1e6, 3),
+            "fp32_gcells": round(gc.cells * Kc / mc_0 / 1e6, 3),
+            "fp64_gcells": round(gc.cells * Kc / mc_64 / 1e6, 3),
+            "speedup_vs_fp64": round(mc_64 / mc_m, 3),
+            "error_vs_fp64": _errors(c_m, c_64),
+            "newton_iters_per_face": {"learned": round(sc_m["newton_iters"] / max(1, sc_m["newton_faces"]), 3),
+                                      "pvrs": round(sc_0["newton_iters"] / max(1, sc_0["newton_faces"]), 3)},
+            "learned_faces": sc_m["mlp_faces"],
+        }
+        log(f"exact+learned 4096: {extra['exact_learned_4096x2048']}")
+        del c_m, c_64
+
+        # -- batched sweep: 64 instances, one persistent launch per step --------
+        from asteriy_2d_amd import batch as B
+        inst = B.sweep(64, seed=5)
+        Kb = 2000
+        bt = B.Batch([x[0] for x in inst], HLLC, [(x[1], x[2]) for x in inst])
+        bt.autotune(reps=3)  # leaves state and time unchanged
+        mb = _timed(bt, Kb, bt.torch_stream())
+        bcells = bt.cells
+        errs = []
+        for k, x in enumerate(inst[:16]):  # per-instance FP64 reference on a subset
+            sim = Simulation(x[0], HLLC, "f64", x[1], x[2])
+            sim.run(Kb).sync()
+            errs.append(_l1(bt.state(k)[0][..., 0], sim.state()[0][..., 0]))
+            sim.close()
+        bt.close()
+        extra["batch_64"] = {
+            "instances": 64, "steps": Kb, "cells": bcells,
+            "gcells": round(bcells * Kb / mb / 1e6, 3),
+            "l1_rho_vs_fp64": {"instances_checked": len(errs), "median": float(np.median(errs)),
+                               "max": float(np.max(errs))},
+        }
+        log(f"batch: {extra['batch_64']}")
 
     cpu = None
     if not a.no_cpu:
@@ -231,7 +333,7 @@ def main(argv=None):
                    "grid": [N_GRID, M_GRID], "method": "hllc", "cfl": 0.3, "gamma": 5 / 3},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": _traffic("step_f32_hllc_2048x1024")},
         "cpu_baseline": cpu,
         **extra,
     }
