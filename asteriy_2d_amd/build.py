@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libasteriy.so")
+EXE = os.path.join(HERE, "asteriy_run")
 ARCH = os.environ.get("AST_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
@@ -68,8 +69,17 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the host driver (the reference's main() over the C-ABI)
+    drv = os.path.join(CSRC, "driver.cpp")
+    if os.path.exists(drv):
+        cmd = [_hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", drv, "-o", EXE + ".tmp",
+               f"-L{HERE}", "-lasteriy", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(EXE + ".tmp", EXE)
     if verbose:
-        print(f"built {LIB} from {len(objs)} sources")
+        print(f"built {LIB} from {len(objs)} sources and {EXE}")
     return LIB
 
 

@@ -85,11 +85,12 @@ AST_DEV float newton_f32(const ExactCtx& e, float enI, float pI, float rI, float
     pn = p - (fI + fII - (enI - enII)) * frcp(dI + dII);
     // the reference's rule |pn - p| < tol*pee, floored at the FP32
     // resolution of the residual (pee can lie far below p* in strong
-    // shocks, where the residual's rounding moves p by ~2 ulp), and
-    // stopped once the steps stop shrinking (the rounding floor)
+    // shocks, where the residual's rounding moves p by ~2 ulp); close to
+    // the root (|dp| < 1e-4 p) it also stops once the steps stop shrinking,
+    // i.e. at the rounding floor
     const float d = fabsf(pn - p);
     if (!(d >= fmaxf(kNewtonTol32 * pee, 2e-6f * fabsf(pn)))) break;
-    if (k > 2 && !(d < 0.5f * dprev)) break;
+    if (k > 2 && d < 1e-4f * fabsf(pn) && !(d < 0.5f * dprev)) break;
     dprev = d;
     if (k >= AST_ITER_CAP) { capped = true; break; }
   }

@@ -84,3 +84,14 @@ def test_step_rejects_bad_arguments_without_gpu(lib):
     rc = L.ast_flux_f64(7, 10, ctypes.c_void_p(8), 1.0, 0.0, 1.0, 1.4, 0.5, ctypes.c_void_p(8),
                         None, None)
     assert rc == 1
+
+
+def test_driver_runs_without_gpu_as_usage_only():
+    import subprocess
+    from asteriy_2d_amd import build
+    if not os.path.exists(build.EXE):
+        build.build()
+    r = subprocess.run([build.EXE, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--problem" in r.stdout
+    r = subprocess.run([build.EXE, "--problem", "nope"], capture_output=True, text=True)
+    assert r.returncode == 2
