@@ -40,7 +40,7 @@ AST_DEV void batch_item(const BatchInst& I, int local, int from, double* red) {
   const double tmin = strip_of_wave<METHOD, GEOM, false>(
       I.g, I.c, I.ex, I.plane[2 * from], I.plane[2 * from + 1], I.plane[2 * (1 - from)],
       I.plane[2 * (1 - from) + 1], I.scal[1], local % I.nbands, local / I.nbands, I.krows, st);
-  block_min_to<NT>(tmin, red, I.scal);
+  block_min_uniform<NT>(tmin, red, I.scal, 0);
 }
 
 template <int METHOD>
@@ -166,6 +166,10 @@ extern "C" int ast_batch_create(int count, const ast_grid_t* grids, int method, 
     if (!(grids[k].ho_lo == -INFINITY && grids[k].ho_hi == INFINITY))
       return ast_fail(AST_ERR_ARG, "ast_batch_create: instance %d: the batch reconstructs every face "
                       "(ho_lo = -inf, ho_hi = +inf)", k);
+    if ((rc = check_positive_bc(grids[k])) != AST_OK) return rc;
+    if (!strip_fits(grids[k], 16))
+      return ast_fail(AST_ERR_SIZE, "ast_batch_create: instance %d: %d columns exceed one buffer range",
+                      k, grids[k].N);
     floats += (size_t)grids[k].N * grids[k].M * 2 * 4;
   }
   if (ast_device_count() < 1) return ast_fail(AST_ERR_NOGPU, "ast_batch_create: no HIP device");
@@ -185,7 +189,7 @@ extern "C" int ast_batch_create(int count, const ast_grid_t* grids, int method, 
     delete B;
     return ast_hip_fail(e, "ast_batch_create: allocation");
   }
-  std::vector<double> init(4 * count);
+  std::vector<double> init(4 * count, 0.0);
   size_t off = 0;
   for (int k = 0; k < count; k++) {
     BatchInst& I = B->h[k];
@@ -204,7 +208,7 @@ extern "C" int ast_batch_create(int count, const ast_grid_t* grids, int method, 
     init[4 * k + 2] = 0.0;
     init[4 * k + 3] = 0.0;
   }
-  AST_HIP_CHECK(hipMemcpy(B->scal, init.data(), sizeof(double) * 4 * count, hipMemcpyHostToDevice));
+  AST_HIP_CHECK(hipMemcpy(B->scal, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice));
   // co-resident capacity: workgroups per CU x CUs
   int dev = 0, cus = 256, per_cu = 1;
   (void)hipGetDevice(&dev);
@@ -301,7 +305,7 @@ extern "C" int ast_batch_get_time(ast_batch_t* B, int k, double* T, double* T_do
 extern "C" int ast_batch_autotune(ast_batch_t* B, int reps) {
   if (!B) return ast_fail(AST_ERR_ARG, "ast_batch_autotune: null batch");
   if (reps < 1) reps = 5;
-  static const int cand[] = {4, 6, 8, 12, 16};
+  static const int cand[] = {4, 8, 12, 16};  // multiples of 4 take the fast strip path
   std::vector<double> saved(4 * B->count);
   AST_HIP_CHECK(hipStreamSynchronize(B->stream));
   AST_HIP_CHECK(hipMemcpy(saved.data(), B->scal, sizeof(double) * saved.size(), hipMemcpyDeviceToHost));

@@ -8,6 +8,7 @@
 
 #include "../../include/asteriy.h"
 #include "ref_flux.hpp"
+#include "time_slots.h"
 
 // ---- host side: status + message of the last failure (api.cpp) ----------
 int ast_fail(int code, const char* fmt, ...);
@@ -128,6 +129,26 @@ AST_DEV void block_min_to(double tmin, double* red, double* T) {
     for (int i = 1; i < NW; i++) t = fmin(t, red[i]);
     if (t < INFINITY)
       atomicMin(reinterpret_cast<unsigned long long*>(T), (unsigned long long)__double_as_longlong(t));
+  }
+}
+
+// block_min_to for a tmin that is already uniform across each wave.  With
+// nslots > 0 workgroup b lowers slot (b mod nslots) of the time block
+// (time_slots.h) instead of *T, so that no single address takes every
+// workgroup's atomic; the time advance folds the slots (funk_time_slots).
+template <int NT>
+AST_DEV void block_min_uniform(double tmin, double* red, double* T, int nslots) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = tmin;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+#pragma unroll
+    for (int i = 1; i < NW; i++) t = fmin(t, red[i]);
+    double* dst = nslots > 0 ? T + kSlot0 + kSlotStride * (int)(blockIdx.x % (unsigned)nslots) : T;
+    if (t < INFINITY)
+      atomicMin(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)__double_as_longlong(t));
   }
 }
 

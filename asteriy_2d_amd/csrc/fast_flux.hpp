@@ -26,6 +26,9 @@ struct FluxF {
 
 AST_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 AST_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// min / max as one v_med3_f32 against -inf / +inf (no operand canonicalisation)
+AST_DEV float fmin3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -INFINITY); }
+AST_DEV float fmax3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, INFINITY); }
 
 // Physical flux of one side in the face frame.
 AST_DEV FluxF phys_flux(const PrimF& q, float E) {
@@ -39,10 +42,10 @@ AST_DEV FluxF phys_flux(const PrimF& q, float E) {
 
 // HLL with the reference's Davis speeds, kernel.cu:1436-1443, 1505-1519.
 AST_DEV FluxF hll_f32(const PrimF& L, const PrimF& R, float g, float ig1, float& UU) {
-  const float c2 = fmaxf(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
+  const float c2 = fmax3(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
   const float cm = fsqrt(c2);
-  const float DL = fminf(L.u, R.u) - cm;
-  const float DR = fmaxf(L.u, R.u) + cm;
+  const float DL = fmin3(L.u, R.u) - cm;
+  const float DR = fmax3(L.u, R.u) + cm;
   UU = fmaxf(fabsf(DL), fabsf(DR));
   const float EL = L.p * ig1 + 0.5f * L.r * (L.u * L.u + L.v * L.v);
   const float ER = R.p * ig1 + 0.5f * R.r * (R.u * R.u + R.v * R.v);
@@ -64,38 +67,51 @@ AST_DEV FluxF hll_f32(const PrimF& L, const PrimF& R, float g, float ig1, float&
 }
 
 // HLLC_2d_Korolkov_b_s (metod 2, W = 0, nul_potok = false).
+//
+// The subsonic star flux F*_K = F_K + S_K (U*_K - U_K) of the reference
+// (Gas_Dinamic.cu:1739-1802) is evaluated in its contact form, which is the
+// same expression rearranged with rho*_K (S_K - S_M) = rho_K (S_K - u_K):
+//   mass      rho*_K S_M
+//   normal    rho*_K S_M^2 + p*      (p* = p_K + rho_K (S_K - u_K)(S_M - u_K))
+//   tangent   rho*_K S_M v_K          (= v_K x mass flux, also when supersonic)
+//   energy    S_M (E*_K + p*)
+// with E*_K = (E_K (S_K - u_K) + p* S_M - p_K u_K) / (S_K - S_M).  The
+// reference's energy uses the average of the two sides' p*, equal to p* in
+// exact arithmetic (that equality is what defines S_M).
 AST_DEV FluxF hllc_f32(const PrimF& L, const PrimF& R, float g, float ig1, float& UU) {
-  const float c2 = fmaxf(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
+  const float c2 = fmax3(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
   const float cm = fsqrt(c2);
-  const float SL = fminf(L.u, R.u) - cm;                 // Gas_Dinamic.cu:1636
-  const float SR = fmaxf(L.u, R.u) + cm;                 // Gas_Dinamic.cu:1637
+  const float SL = fmin3(L.u, R.u) - cm;                 // Gas_Dinamic.cu:1636
+  const float SR = fmax3(L.u, R.u) + cm;                 // Gas_Dinamic.cu:1637
   UU = fmaxf(fabsf(SL), fabsf(SR));
   const float suL = SL - L.u, suR = SR - R.u;
   const float mL = L.r * suL, mR = R.r * suR;
   const float SM = (mR * R.u - mL * L.u - R.p + L.p) * frcp(mR - mL);  // :1645-1646
   // side whose flux (or star flux) is taken, Gas_Dinamic.cu:1686-1785
   const bool supL = SL >= 0.f, supR = SR <= 0.f;
-  const bool left = supL || (!supR && SM >= 0.f);
-  const PrimF K = left ? L : R;
+  const bool left = supL | (!supR & (SM >= 0.f));
+  const float rK = left ? L.r : R.r, pK = left ? L.p : R.p;
+  const float uK = left ? L.u : R.u, vK = left ? L.v : R.v;
   const float SK = left ? SL : SR;
-  const float suK = left ? suL : suR;
   const float mK = left ? mL : mR;
-  const float EK = K.p * ig1 + 0.5f * K.r * (K.u * K.u + K.v * K.v);
-  FluxF f = phys_flux(K, EK);
-  // star state of side K, averaged ptz (Gas_Dinamic.cu:1739-1744, 1772-1802)
-  const float ptz = 0.5f * ((R.p + mR * (SM - R.u)) + (L.p + mL * (SM - L.u)));
+  const float EK = pK * ig1 + 0.5f * rK * (uK * uK + vK * vK);
+  // physical flux of side K (supersonic faces)
+  const float fm = rK * uK;
+  const float fun = fm * uK + pK;
+  const float fe = (EK + pK) * uK;
+  // star flux of side K (subsonic faces; non-finite star terms of a
+  // supersonic face are discarded by the selects, no branch)
+  const float ps = pK + mK * (SM - uK);
   const float inv = frcp(SK - SM);
   const float rs = mK * inv;
-  const float Es = (EK * suK + ptz * SM - K.p * K.u) * inv;
-  const float dr = rs - K.r;
-  // subsonic: F*_K = F_K + S_K (U*_K - U_K); supersonic: F_K (selects, no
-  // branch -- the star terms may be non-finite there and are discarded)
-  const bool sub = !(supL || supR);
-  const float sk = SK;
-  f.m = sub ? f.m + sk * dr : f.m;
-  f.un = sub ? f.un + sk * (rs * SM - K.r * K.u) : f.un;
-  f.ut = sub ? f.ut + sk * dr * K.v : f.ut;
-  f.e = sub ? f.e + sk * (Es - EK) : f.e;
+  const float Es = (EK * (SK - uK) + ps * SM - pK * uK) * inv;
+  const float sm = rs * SM;
+  const bool sub = !(supL | supR);
+  FluxF f;
+  f.m = sub ? sm : fm;
+  f.un = sub ? sm * SM + ps : fun;
+  f.e = sub ? SM * (Es + ps) : fe;
+  f.ut = f.m * vK;
   return f;
 }
 

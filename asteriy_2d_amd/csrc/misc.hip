@@ -14,6 +14,28 @@ __global__ void funk_time_kernel(double* T, double* T_do, double* TT, int* i) {
   }
 }
 
+// funk_time on the runtime's time block with a spread dt minimum: T is the
+// minimum of the slots (time_slots.h), which are re-armed to 1e7 with T.
+// One wave; lane k folds slot k.
+__global__ void funk_time_slots_kernel(double* T, int nslots) {
+  const int k = threadIdx.x;
+  double t = INFINITY;
+  if (k < nslots) {
+    double* slot = T + kSlot0 + kSlotStride * k;
+    t = *slot;
+    *slot = 10000000;
+  }
+  t = wave_min(t);
+  if (k == 0) {
+    t = fmin(t, *T);  // a step launched without slots lowers T itself
+    T[1] = t;
+    T[2] = T[2] + t;
+    T[0] = 10000000;
+    int* i = reinterpret_cast<int*>(T + 3);
+    *i = *i + 1;
+  }
+}
+
 __global__ void f64_to_f32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x)
@@ -92,6 +114,16 @@ __global__ void sums_pass2(const double* work, double* out) {
 }
 
 }  // namespace ast
+
+// The runtime's time advance over a time block with `nslots` dt slots.
+int ast_funk_time_slots(double* T, int nslots, void* stream) {
+  if (!T || nslots < 0 || nslots > ast::kMaxSlots)
+    return ast_fail(AST_ERR_ARG, "ast_funk_time_slots: bad argument");
+  hipLaunchKernelGGL(ast::funk_time_slots_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, T,
+                     nslots);
+  AST_LAUNCH_CHECK("ast_funk_time_slots launch");
+  return AST_OK;
+}
 
 extern "C" int ast_funk_time(double* T, double* T_do, double* TT, int* i, void* stream) {
   if (!T || !T_do || !TT) return ast_fail(AST_ERR_ARG, "ast_funk_time: null pointer");

@@ -34,7 +34,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // k-slot s (0..31) of a B/A operand <-> hidden unit: lane group q = s/8
 // holds the units its accumulators hold after the previous layer, rows
 // 4q..4q+3 of output tile 0 and of output tile 1.
-AST_DEV __host__ inline int slot_unit(int s) {
+AST_DEV __host__ int slot_unit(int s) {
   const int q = s >> 3, v = s & 7;
   return v < 4 ? 4 * q + v : 16 + 4 * q + (v - 4);
 }

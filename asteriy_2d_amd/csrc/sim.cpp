@@ -17,12 +17,14 @@
 #include <vector>
 
 #include "../../include/asteriy.h"
+#include "time_slots.h"
 
 int ast_fail(int code, const char* fmt, ...);
 int ast_hip_fail(hipError_t e, const char* where);
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, void* stream);
+                      ast_stats_t* stats, int krows, int nslots, void* stream);
+int ast_funk_time_slots(double* T, int nslots, void* stream);
 extern "C" int ast_get_tile_rows(void);
 
 #define CK(call)                                              \
@@ -39,7 +41,8 @@ struct ast_sim {
   size_t cells;
   void* plane[4];  // s0, u0, s1, u1
   int cur;         // 0: state in (s0,u0); 1: in (s1,u1)
-  double* scal;    // T, T_do, TT (+ int i in the 4th word)
+  double* scal;    // time block: T, T_do, TT (+ int i in the 4th word), then
+                   // the FP32 step's dt slots (time_slots.h)
   ast_stats_t* stats;
   const ast_mlp_t* mlp;
   hipStream_t stream;
@@ -61,10 +64,12 @@ static int launch_step(ast_sim_t* S, int from) {
   if (S->precision == 0)
     rc = ast_step_f64(&S->g, (const double*)s, (const double*)u, (double*)s2, (double*)u2, T,
                       T + 1, T + 2, S->method, S->stats, S->stream);
-  else
+  else  // dt spread over the slots of the time block, folded by the time advance
     rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
-                           T + 1, S->method, S->mlp, S->stats, S->krows, S->stream);
+                           T + 1, S->method, S->mlp, S->stats, S->krows, ast::kMaxSlots,
+                           S->stream);
   if (rc != AST_OK) return rc;
+  if (S->precision == 1) return ast_funk_time_slots(T, ast::kMaxSlots, S->stream);
   return ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
 }
 
@@ -96,7 +101,7 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
   const size_t bytes = S->cells * 2 * elem(S);
   hipError_t e = hipSuccess;
   for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipMalloc(&S->plane[i], bytes);
-  if (e == hipSuccess) e = hipMalloc((void**)&S->scal, 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&S->scal, ast::kTimeBlockDoubles * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&S->stats, sizeof(ast_stats_t));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -107,9 +112,12 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
     delete tmp;
     return ast_hip_fail(e, "ast_sim_create: allocation");
   }
-  // kernel.cu:3321-3324: T = 1e7, T_do = 1e-9, TT = 0, i = 0
-  double init[4] = {1e7, 1e-9, 0.0, 0.0};
-  CK(hipMemcpy(S->scal, init, sizeof init, hipMemcpyHostToDevice));
+  // kernel.cu:3321-3324: T = 1e7, T_do = 1e-9, TT = 0, i = 0; dt slots 1e7
+  std::vector<double> init(ast::kTimeBlockDoubles, 0.0);
+  init[0] = 1e7;
+  init[1] = 1e-9;
+  for (int k = 0; k < ast::kMaxSlots; k++) init[ast::kSlot0 + ast::kSlotStride * k] = 1e7;
+  CK(hipMemcpy(S->scal, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice));
   CK(hipMemset(S->stats, 0, sizeof(ast_stats_t)));
   *out = S;
   return AST_OK;
@@ -142,6 +150,11 @@ extern "C" int ast_sim_set_state(ast_sim_t* S, const double* s, const double* u)
       a[k] = (float)s[k];
       b[k] = (float)u[k];
     }
+    // the FP32 step needs positive densities and pressures
+    for (size_t k = 0; k < n; k++)
+      if (!(a[k] > 0.f))
+        return ast_fail(AST_ERR_ARG, "ast_sim_set_state: cell %zu has %s %g (the FP32 step needs "
+                        "rho > 0 and p > 0)", k / 2, (k & 1) ? "p" : "rho", (double)a[k]);
     CK(hipMemcpy(S->plane[0], a.data(), n * sizeof(float), hipMemcpyHostToDevice));
     CK(hipMemcpy(S->plane[1], b.data(), n * sizeof(float), hipMemcpyHostToDevice));
   }
@@ -237,7 +250,7 @@ extern "C" void* ast_sim_stream(ast_sim_t* S) { return S ? (void*)S->stream : nu
 extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, int reps,
                                 int* best_rows, float* best_us) {
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_autotune: null sim");
-  static const int kDefault[] = {3, 4, 5, 6, 7, 8, 10, 12, 16};
+  static const int kDefault[] = {4, 8, 12, 16};  // multiples of 4 take the fast strip path
   if (!candidates || ncand <= 0) {
     candidates = kDefault;
     ncand = (int)(sizeof kDefault / sizeof kDefault[0]);
@@ -265,7 +278,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
       int rc = S->precision == 1
                    ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                        (float*)u2, S->scal, S->scal + 1, S->method, S->mlp,
-                                       nullptr, kr, S->stream)
+                                       nullptr, kr, 0, S->stream)
                    : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
                                   (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
                                   nullptr, S->stream);

@@ -19,8 +19,10 @@
 //    by coordinate differences and multiplies by the half spacing: the
 //    same number in exact arithmetic, kernel.cu:35-50).
 //  * dt: each lane keeps the largest |wave speed| of its x- and y-faces;
-//    dt = krit*min(dx/max_x, dy/max_y) in FP64, reduced by wave shuffles and
-//    one 64-bit atomicMin per workgroup (deterministic).
+//    the wave reduces them (shuffles), dt = krit*min(dx/max_x, dy/max_y) in
+//    FP64, one 64-bit atomicMin per workgroup (deterministic).  The runtime
+//    spreads these atomics over 64 slots (one address would serialise
+//    them) and folds the slots when it advances the time.
 //  * the exact solver runs in FP32 with its p* solve promoted to FP64 near
 //    vacuum and in strong rarefactions, optionally seeded by the learned
 //    guess (exact_f32.hpp).
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   float2* __restrict__ u2, double* T,
                                                   const double* __restrict__ T_do,
                                                   ast_stats_t* stats, int nbands, int krows,
-                                                  int nwg) {
+                                                  int nwg, int nslots) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
   const int wg = xcd_tile(blockIdx.x, nwg);
@@ -45,8 +47,8 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   st.clear();
   const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, *T_do,
                                                           wg % nbands, wg / nbands, krows, st);
-  block_min_to<NT>(tmin, red, T);
   if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<NT>(st, redi, stats);
+  block_min_uniform<NT>(tmin, red, T, nslots);
 }
 
 }  // namespace f32
@@ -67,14 +69,14 @@ extern "C" int ast_get_tile_rows(void) { return g_krows; }
 template <int METHOD, int GEOM, bool WINDOW>
 static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
                       const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
-                      ast_stats_t* stats, int krows, hipStream_t stream) {
+                      ast_stats_t* stats, int krows, int nslots, hipStream_t stream) {
   const int nbands = (g.N + WOUT - 1) / WOUT;
   const int nstrips = (g.M + krows - 1) / krows;
   const int nsg = (nstrips + WPB - 1) / WPB;
   const long long nwg = (long long)nbands * nsg;
   if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
   hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW>), dim3((unsigned)nwg), dim3(NT), 0,
-                     stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg);
+                     stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg, nslots);
   AST_LAUNCH_CHECK("ast_step_f32 launch");
   return AST_OK;
 }
@@ -82,19 +84,20 @@ static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, 
 template <int METHOD>
 static int dispatch_geom(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, bool window,
                          const float2* s, const float2* u, float2* s2, float2* u2, double* T,
-                         const double* T_do, ast_stats_t* stats, int krows, hipStream_t st) {
+                         const double* T_do, ast_stats_t* stats, int krows, int nslots,
+                         hipStream_t st) {
   const bool axi = g.geometry == AST_AXISYMMETRIC;
   if (axi) {
-    return window ? launch_f32<METHOD, AST_AXISYMMETRIC, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st)
-                  : launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st);
+    return window ? launch_f32<METHOD, AST_AXISYMMETRIC, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
+                  : launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
   }
-  return window ? launch_f32<METHOD, AST_PLANAR, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st)
-                : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, st);
+  return window ? launch_f32<METHOD, AST_PLANAR, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
+                : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
 }
 
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, void* stream) {
+                      ast_stats_t* stats, int krows, int nslots, void* stream) {
   int rc = ast_grid_check(g);
   if (rc != AST_OK) return rc;
   if (!s || !u || !s2 || !u2 || !T || !T_do)
@@ -102,6 +105,12 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   if (s == s2 || u == u2 || s == u2 || u == s2)
     return ast_fail(AST_ERR_ARG, "ast_step_f32: output planes must not alias the input planes");
   if (krows < 1) return ast_fail(AST_ERR_ARG, "ast_step_f32: krows %d", krows);
+  if (nslots < 0 || nslots > kMaxSlots) return ast_fail(AST_ERR_ARG, "ast_step_f32: %d dt slots", nslots);
+  if (!strip_fits(*g, krows))
+    return ast_fail(AST_ERR_SIZE, "ast_step_f32: %d columns x %d rows per strip exceed one buffer range",
+                    g->N, krows + 4);
+  rc = check_positive_bc(*g);
+  if (rc != AST_OK) return rc;
   const Consts c = make_consts(*g);
   const ExactCtx ex = make_exact_ctx(g->gamma, mlp);
   const bool window = !(g->ho_lo == -INFINITY && g->ho_hi == INFINITY);
@@ -111,9 +120,9 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   float2* S2 = (float2*)s2;
   float2* U2 = (float2*)u2;
   switch (method) {
-    case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, st);
-    case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, st);
-    case AST_EXACT: return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, st);
+    case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+    case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+    case AST_EXACT: return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
     default: return ast_fail(AST_ERR_ARG, "ast_step_f32: unknown method %d", method);
   }
 }
@@ -122,5 +131,5 @@ extern "C" int ast_step_f32(const ast_grid_t* g, const float* s, const float* u,
                             float* u2, double* T, const double* T_do, double* TT, int method,
                             const ast_mlp_t* mlp, ast_stats_t* stats, void* stream) {
   (void)TT;
-  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, stream);
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, 0, stream);
 }

@@ -13,54 +13,63 @@ constexpr int WOUT = 60;  // output columns per wave
 constexpr int WPB = 4;    // waves per workgroup (stacked strips)
 constexpr int NT = 64 * WPB;
 
+// Buffer resource word 3 for raw (stride 0) buffers on the gfx9 family:
+// DATA_FORMAT = 32 bit, no swizzle, no thread-id offset.
+constexpr int kBufWord3 = 0x00020000;
+// A byte offset beyond every range used here: a store at it (plus a row
+// offset below 2^31) is dropped by the buffer range check (range sizes stay
+// below 2^31, see kMaxStripBytes).
+constexpr unsigned kOffDrop = 0x80000000u;
+// Largest byte range one buffer resource of a strip covers.
+constexpr long long kMaxStripBytes = 0x7fff0000LL;
+
 struct Q {
   float r, p, vx, vy;
 };
 
-// lane i <- lane i-1 (lane 0 keeps its own value)
+// lane i <- lane i-1; lane 0 reads 0.  Lane 0 is a halo lane: its results
+// never reach an output lane (the zero fill lets the shift fold into the
+// consuming instruction as a DPP operand).
 AST_DEV float shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+// lane i <- lane i+1; lane 63 reads 0 (halo lane, as above)
+AST_DEV float shl1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+// lane i <- lane i-1; lane 0 keeps its own value.  For the left state of a
+// face: lane 0's (unused) face then stays an ordinary physical face, which
+// matters to the iterative exact solver (a zero state is a vacuum problem
+// that would hold the whole wave in its slowest path).
+AST_DEV float shr1_keep(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
                                                     0xf, 0xf, false));
 }
-// lane i <- lane i+1 (lane 63 keeps its own value)
-AST_DEV float shl1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
-                                                    0xf, 0xf, false));
-}
 AST_DEV Q shr1(const Q& q) { return Q{shr1(q.r), shr1(q.p), shr1(q.vx), shr1(q.vy)}; }
-AST_DEV Q shl1(const Q& q) { return Q{shl1(q.r), shl1(q.p), shl1(q.vx), shl1(q.vy)}; }
-AST_DEV FluxF shl1(const FluxF& f) { return FluxF{shl1(f.m), shl1(f.e), shl1(f.un), shl1(f.ut)}; }
+AST_DEV Q shr1_keep(const Q& q) {
+  return Q{shr1_keep(q.r), shr1_keep(q.p), shr1_keep(q.vx), shr1_keep(q.vy)};
+}
+AST_DEV Q sub(const Q& a, const Q& b) { return Q{a.r - b.r, a.p - b.p, a.vx - b.vx, a.vy - b.vy}; }
 
 // half the minmod slope, 0.5*minmod(a, b) (kernel.cu:22-33): minmod(a, b) is
 // the median of {a, b, 0} -- one v_med3_f32.
 AST_DEV float hmm(float a, float b) { return 0.5f * __builtin_amdgcn_fmed3f(a, b, 0.f); }
 
-// Face values at +-1/2 of a cell from its half slopes, with the positivity
-// fallback of kernel.cu:2706-2788 (rho, p <= 0 -> the cell value).
-AST_DEV void face_values(const Q& c, const Q& h, Q& plus, Q& minus) {
-  plus.r = c.r + h.r;
-  minus.r = c.r - h.r;
-  plus.p = c.p + h.p;
-  minus.p = c.p - h.p;
-  plus.vx = c.vx + h.vx;
-  minus.vx = c.vx - h.vx;
-  plus.vy = c.vy + h.vy;
-  minus.vy = c.vy - h.vy;
-  if (plus.r <= 0.f) plus.r = c.r;
-  if (minus.r <= 0.f) minus.r = c.r;
-  if (plus.p <= 0.f) plus.p = c.p;
-  if (minus.p <= 0.f) minus.p = c.p;
-}
-
-AST_DEV Q half_slopes(const Q& a, const Q& c, const Q& b) {
-  return Q{hmm(c.r - a.r, b.r - c.r), hmm(c.p - a.p, b.p - c.p), hmm(c.vx - a.vx, b.vx - c.vx),
-           hmm(c.vy - a.vy, b.vy - c.vy)};
+// Face values at +-1/2 of a cell from its neighbour differences dl = c - a,
+// dr = b - c.  The reference's positivity fallback (kernel.cu:2706-2788:
+// rho, p <= 0 -> the cell value) never fires for positive cell states: with
+// 0 <= |mm| <= |dl|, |dr| of the same sign, c + mm/2 and c - mm/2 lie between
+// c/2 and the neighbours' means (also after rounding), so the fast path
+// requires rho, p > 0 in every cell instead of testing each face value.
+AST_DEV void face_values(const Q& c, const Q& dl, const Q& dr, Q& plus, Q& minus) {
+  const float hr = hmm(dl.r, dr.r), hp = hmm(dl.p, dr.p);
+  const float hx = hmm(dl.vx, dr.vx), hy = hmm(dl.vy, dr.vy);
+  plus = Q{c.r + hr, c.p + hp, c.vx + hx, c.vy + hy};
+  minus = Q{c.r - hr, c.p - hp, c.vx - hx, c.vy - hy};
 }
 
 // Cell (n, m) in FP32 with the ghost rules; corner/out-of-halo positions are
 // clamped into the grid (never read by an output cell, kept finite).
-// `interior` (wave-uniform: every lane's column and the row lie inside the
-// grid) takes the plain load without any ghost logic.
 AST_DEV Q load(const ast_grid_t& g, const float2* __restrict__ s, const float2* __restrict__ u,
                int n, int m, bool cols_in) {
   float2 a, b;
@@ -81,6 +90,23 @@ AST_DEV Q load(const ast_grid_t& g, const float2* __restrict__ s, const float2* 
     fetch_cell(g, s, u, n, m, a, b);
   }
   return Q{a.x, a.y, b.x, b.y};
+}
+
+// ---- raw buffer access ----------------------------------------------------
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+AST_DEV __amdgpu_buffer_rsrc_t make_rsrc(const void* base, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kBufWord3);
+}
+AST_DEV float2 bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+AST_DEV void bstore(float a, float b, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  u32x2 v;
+  v.x = __float_as_uint(a);
+  v.y = __float_as_uint(b);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)voff, (int)soff, 0);
 }
 
 struct Consts {
@@ -114,15 +140,20 @@ static inline StripShape strip_shape(const ast_grid_t& g, int krows) {
   sh.nsg = (nstrips + WPB - 1) / WPB;
   return sh;
 }
-
-// The strip of one wave given its workgroup's (band, strip group): fills
-// StripArgs and runs the fast or the boundary variant.  Returns the lane's
-// dt candidate.
-template <int METHOD, int GEOM, bool WINDOW>
-AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCtx& ex,
-                             const float2* __restrict__ s, const float2* __restrict__ u,
-                             float2* __restrict__ s2, float2* __restrict__ u2, double tdo,
-                             int band, int sgrp, int krows, FaceStats& st);
+// Do the buffer ranges of a strip fit one resource?  (rows krows + 4 of N
+// float2 cells)
+static inline bool strip_fits(const ast_grid_t& g, int krows) {
+  return (long long)(krows + 4) * g.N * 8 <= kMaxStripBytes;
+}
+// The fast path reconstructs without the positivity fallback, which needs
+// positive densities and pressures in every cell and ghost: a Dirichlet side
+// must hold a positive state.
+static inline int check_positive_bc(const ast_grid_t& g) {
+  for (int k = 0; k < 4; k++)
+    if (g.bc[k] == AST_BC_DIRICHLET && !(g.bc_state[k][0] > 0.0 && g.bc_state[k][1] > 0.0))
+      return ast_fail(AST_ERR_ARG, "FP32 step: Dirichlet side %d needs rho > 0 and p > 0", k);
+  return AST_OK;
+}
 
 template <int METHOD>
 AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
@@ -176,110 +207,200 @@ AST_DEV bool frozen_possible(const ast_grid_t& g, int n0, int n1, int m0, int m1
 struct StripArgs {
   int n, m0, m1;
   bool cols_in, out_col, xvote, ho_x, ho_y;
-  float tdV, tdof, dxf, dyf, x, ymin;
+  float ax, ay;   // -T_do/dx, -T_do/dy
+  float tdof, dxf, dyf, x, ymin;
+  unsigned col_off;  // byte offset of the lane's column in a row (interior strips)
+  unsigned st_off;   // the same for the stores, kOffDrop on lanes that write nothing
+  unsigned row_bytes;
 };
 
-// One wave's strip: prologue + the row loop.  FAST (wave-uniform): every
-// load of the strip lies inside the grid, so it is a plain load with no
-// ghost logic; otherwise load() applies the boundary rules.
+template <int J>
+struct Row {
+  static constexpr int w0 = J & 3, w1 = (J + 1) & 3, w2 = (J + 2) & 3, w3 = (J + 3) & 3;
+  static constexpr int e0 = J & 1, e1 = (J + 1) & 1;
+};
+
+// One wave's strip.  FAST (wave-uniform): every load of the strip lies inside
+// the grid, no cell can be frozen and the strip has a multiple of 4 rows, so
+// rows are raw buffer loads at fixed per-lane offsets and no ghost, frozen
+// or row-count logic runs; otherwise load() applies the boundary rules, the
+// frozen tests run per row, and the rows of the last group of four that lie
+// past the strip vote for no dt (their stores fall outside the store range).
+//
+// Rows are processed four per trip with the window held in w[4] (row r in
+// slot (r - m0) & 3): the slide of the window is a change of names, not a
+// copy.  Row m + 3 is fetched while row m is computed; no branch separates
+// the four rows of a trip, so the fetch stays a row ahead of its use.  The
+// stores of the lanes that own no output go to an offset beyond the store
+// range and are dropped by the range check, so every row issues the same
+// memory operations and the load of row m + 3 never waits for a store.
 template <int METHOD, int GEOM, bool WINDOW, bool FAST>
 AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, const ExactCtx& ex,
                        const float2* __restrict__ s, const float2* __restrict__ u,
                        float2* __restrict__ s2, float2* __restrict__ u2, const StripArgs& a,
                        float& UUx, float& UUy, FaceStats& st) {
   const int n = a.n, m0 = a.m0, m1 = a.m1, N = g.N;
-  const float dxf = a.dxf, dyf = a.dyf, x = a.x;
+  const unsigned rb = a.row_bytes;
+  // loads: FAST strips read rows m0-2 .. m1+1 through one resource per plane
+  const size_t b0 = FAST ? (size_t)(m0 - 2) * N : 0;
+  const long long lbytes = FAST ? (long long)(m1 - m0 + 4) * rb : 0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(s + b0, lbytes), ru = make_rsrc(u + b0, lbytes);
   auto LD = [&](int m) -> Q {
     if constexpr (FAST) {
-      const size_t k = (size_t)m * N + n;
-      const float2 p = s[k], q = u[k];
+      const unsigned so = (unsigned)(m - (m0 - 2)) * rb;
+      const float2 p = bload(rs, a.col_off, so), q = bload(ru, a.col_off, so);
       return Q{p.x, p.y, q.x, q.y};
     } else {
       return load(g, s, u, n, m, a.cols_in);
     }
   };
+  // last row a prefetch may name (the boundary variant's load() clamps any
+  // row into the grid)
+  const int mlast = FAST ? m1 + 1 : m1 + 5;
+  // stores: rows m0 .. m1-1
+  const size_t sb = (size_t)m0 * N;
+  const long long sbytes = (long long)(m1 - m0) * rb;
+  const __amdgpu_buffer_rsrc_t ws = make_rsrc(s2 + sb, sbytes), wu = make_rsrc(u2 + sb, sbytes);
+
+  const bool frz = !FAST && cw.frozen;  // wave-uniform
+
   // prologue: rows m0-2 .. m0+1, the south face of row m0
-  const Q qm2 = LD(m0 - 2);
-  const Q qm1 = LD(m0 - 1);
-  Q q0 = LD(m0);
-  Q q1 = LD(m0 + 1);
-  Q nvm1, svm1, nv0, sv0;
-  face_values(qm1, half_slopes(qm2, qm1, q0), nvm1, svm1);
-  face_values(q0, half_slopes(qm1, q0, q1), nv0, sv0);
-  float uu;
-  FluxF Gs = face<METHOD, WINDOW, false>(nvm1, sv0, qm1, q0, a.ho_y, c, uu, ex, st);
-  bool fr0 = frozen_at(cw, x, a.ymin + m0 * dyf);
-  const bool frm1 = frozen_at(cw, x, a.ymin + (m0 - 1) * dyf);
-  if (a.out_col && !(fr0 && frm1)) UUy = fmaxf(UUy, uu);
+  Q w[4], nv[2], dd[2];
+  FluxF G[2];
+  bool fr[2] = {false, false};
+  {
+    const Q qa = LD(m0 - 2), qb = LD(m0 - 1);
+    w[0] = LD(m0);
+    w[1] = LD(m0 + 1);
+    w[2] = LD(m0 + 2);
+    const Q d0 = sub(qb, qa), d1 = sub(w[0], qb);
+    dd[0] = sub(w[1], w[0]);
+    Q nvm1, svm1, sv0;
+    face_values(qb, d0, d1, nvm1, svm1);
+    face_values(w[0], d1, dd[0], nv[0], sv0);
+    float uu;
+    G[0] = face<METHOD, WINDOW, false>(nvm1, sv0, qb, w[0], a.ho_y, c, uu, ex, st);
+    if (frz) {
+      fr[0] = frozen_at(cw, a.x, a.ymin + m0 * a.dyf);
+      const bool frm1 = frozen_at(cw, a.x, a.ymin + (m0 - 1) * a.dyf);
+      if (!(fr[0] && frm1)) UUy = fmaxf(UUy, uu);
+    } else {
+      UUy = fmaxf(UUy, uu);
+    }
+  }
 
-  // rows are fetched one iteration ahead of their use (latency hiding)
-  Q qnext = LD(m0 + 2);
-  size_t k = (size_t)m0 * N + n;
-  for (int m = m0; m < m1; m++, k += N) {
-    const Q q2 = qnext;
-    if (m + 1 < m1) qnext = LD(m + 3);
+  // results of the previous row, stored after this row's fetch is issued
+  // (memory operations retire in order: a store issued before the fetch of
+  // row m+3 would make the wait for row m+2 wait for the store too).  The
+  // first pending store goes to the drop offset.
+  Q pend = w[0];
+  unsigned pend_vo = kOffDrop;
+
+  auto row = [&](auto tag, int m) {
+    using R = decltype(tag);
+    const Q& q0 = w[R::w0];
+    const Q& q1 = w[R::w1];
+    const Q& q2 = w[R::w2];
+    w[R::w3] = LD(min(m + 3, mlast));
+    bstore(pend.r, pend.p, ws, pend_vo, 0);
+    bstore(pend.vx, pend.vy, wu, pend_vo, 0);
+    // keep the fetch of row m+3 here, a row ahead of its use (the
+    // scheduler would otherwise sink it towards its first use)
+    if constexpr (FAST) __builtin_amdgcn_sched_barrier(0);
+    // y: face values of row m+1, the north face of row m
+    const Q d2 = sub(q2, q1);
     Q nv1, sv1;
-    face_values(q1, half_slopes(q0, q1, q2), nv1, sv1);
-    const float y = a.ymin + m * dyf;
-    const bool fr1 = frozen_at(cw, x, y + dyf);
-    // north face of row m
-    const FluxF Gn = face<METHOD, WINDOW, false>(nv0, sv1, q0, q1, a.ho_y, c, uu, ex, st);
-    if (a.out_col && !(fr0 && fr1)) UUy = fmaxf(UUy, uu);
-    // west face of row m: left state = east value of lane-1
-    const Q ql = shr1(q0), qr = shl1(q0);
+    face_values(q1, dd[R::e0], d2, nv1, sv1);
+    dd[R::e1] = d2;
+    const float y = a.ymin + m * a.dyf;
+    const bool live = FAST || m < m1;  // wave-uniform
+    float uu;
+    const FluxF Gn = face<METHOD, WINDOW, false>(nv[R::e0], sv1, q0, q1, a.ho_y, c, uu, ex, st);
+    nv[R::e1] = nv1;
+    // x: face values of row m, its west face (left state = east value of
+    // lane-1); the east face flux is the west face flux of lane+1.  The
+    // zero-filled shifts leave lane 0's slopes and lane 63's east flux
+    // meaningless (finite, positive face values): both are halo lanes.
+    const Q ql = shr1(q0);
+    const Q dl = sub(q0, ql);
+    const Q dr = Q{shl1(q0.r) - q0.r, shl1(q0.p) - q0.p, shl1(q0.vx) - q0.vx, shl1(q0.vy) - q0.vy};
     Q ev, wv;
-    face_values(q0, half_slopes(ql, q0, qr), ev, wv);
-    const Q evl = shr1(ev);
+    face_values(q0, dl, dr, ev, wv);
     float ux;
-    const FluxF Fw = face<METHOD, WINDOW, true>(evl, wv, ql, q0, a.ho_x, c, ux, ex, st);
-    const FluxF Fe = shl1(Fw);
-    const bool frl = frozen_at(cw, x - dxf, y);
-    if (a.xvote && !(fr0 && frl)) UUx = fmaxf(UUx, ux);
-
-    // outward flux sums, kernel.cu:3027-3032 (x faces: Fe - Fw, y faces: Gn - Gs)
-    const float PSx = (Fe.m - Fw.m) * dyf + (Gn.m - Gs.m) * dxf;
-    const float PSy = (Fe.e - Fw.e) * dyf + (Gn.e - Gs.e) * dxf;
-    const float PUx = (Fe.un - Fw.un) * dyf + (Gn.ut - Gs.ut) * dxf;
-    const float PUy = (Fe.ut - Fw.ut) * dyf + (Gn.un - Gs.un) * dxf;
+    const FluxF Fw = face<METHOD, WINDOW, true>(shr1_keep(ev), wv, ql, q0, a.ho_x, c, ux, ex, st);
+    if (frz) {
+      const bool f0 = fr[R::e0];
+      const bool f1 = frozen_at(cw, a.x, y + a.dyf);
+      const bool fl = frozen_at(cw, a.x - a.dxf, y);
+      if (live && !(f0 && f1)) UUy = fmaxf(UUy, uu);
+      if (live && !(f0 && fl)) UUx = fmaxf(UUx, ux);
+      fr[R::e1] = f1;
+    } else {
+      UUy = fmaxf(UUy, live ? uu : 0.f);
+      UUx = fmaxf(UUx, live ? ux : 0.f);
+    }
+    const FluxF& Gs = G[R::e0];
+    // conservative update, kernel.cu:3027-3032, 3059-3073 (x faces: Fe - Fw,
+    // y faces: Gn - Gs), dU = -T_do/dx (Fe - Fw) - T_do/dy (Gn - Gs)
+    float dr_ = a.ax * (shl1(Fw.m) - Fw.m) + a.ay * (Gn.m - Gs.m);
+    float dE = a.ax * (shl1(Fw.e) - Fw.e) + a.ay * (Gn.e - Gs.e);
+    float dmx = a.ax * (shl1(Fw.un) - Fw.un) + a.ay * (Gn.ut - Gs.ut);
+    float dmy = a.ax * (shl1(Fw.ut) - Fw.ut) + a.ay * (Gn.un - Gs.un);
+    G[R::e1] = Gn;
     const float ke = 0.5f * q0.r * (q0.vx * q0.vx + q0.vy * q0.vy);
-    float dr = -a.tdV * PSx, dmx = -a.tdV * PUx, dmy = -a.tdV * PUy, dE = -a.tdV * PSy;
+    const float mx = q0.r * q0.vx, my = q0.r * q0.vy;
     if constexpr (GEOM == AST_AXISYMMETRIC) {
       // KorolkovSolver.cu:13-32: -(T_do/y) v {rho, rho u, rho v, E + p}
       const float ty = a.tdof * frcp(y);
-      const float rv = q0.r * q0.vy;
-      dr -= ty * rv;
-      dmx -= ty * rv * q0.vx;
-      dmy -= ty * rv * q0.vy;
+      const float tr = ty * my;
+      dr_ -= tr;
+      dmx -= tr * q0.vx;
+      dmy -= tr * q0.vy;
       dE -= ty * q0.vy * (c.gig1 * q0.p + ke);
     }
-    float r2 = q0.r + dr;
+    float r2 = q0.r + dr_;
     r2 = (r2 <= 0.f) ? q0.r : r2;  // kernel.cu:3060-3064
     const float ir = frcp(r2);
-    float vx2 = (q0.r * q0.vx + dmx) * ir;
-    float vy2 = (q0.r * q0.vy + dmy) * ir;
+    float vx2 = (mx + dmx) * ir;
+    float vy2 = (my + dmy) * ir;
     // p' = (E' - rho'|u'|^2/2)(gamma-1), written so that the O(E) terms
     // cancel exactly: E' - KE' = p/(g-1) + dE - (KE' - KE) with
     // KE' - KE = (m.dm + |dm|^2/2 - KE*dr) / rho'.
-    const float mx = q0.r * q0.vx, my = q0.r * q0.vy;
-    const float dke = (mx * dmx + my * dmy + 0.5f * (dmx * dmx + dmy * dmy) - ke * dr) * ir;
+    const float dke = (mx * dmx + my * dmy + 0.5f * (dmx * dmx + dmy * dmy) - ke * dr_) * ir;
     float p2 = q0.p + c.g1 * (dE - dke);
     p2 = (p2 <= 0.f) ? 0.000001f : p2;  // kernel.cu:3070-3073
-    if (fr0) {  // kernel.cu:2317-2324
-      r2 = q0.r; p2 = q0.p; vx2 = q0.vx; vy2 = q0.vy;
+    if (frz && fr[R::e0]) {  // kernel.cu:2317-2324
+      r2 = q0.r;
+      p2 = q0.p;
+      vx2 = q0.vx;
+      vy2 = q0.vy;
     }
-    if (a.out_col) {
-      s2[k] = make_float2(r2, p2);
-      u2[k] = make_float2(vx2, vy2);
-    }
-    // slide the window
-    q0 = q1;
-    q1 = q2;
-    nv0 = nv1;
-    Gs = Gn;
-    fr0 = fr1;
+    // the row offset goes into the VGPR offset, which the range check
+    // covers (the SGPR offset is not range checked): rows past the strip
+    // and non-output lanes fall outside the store range
+    pend = Q{r2, p2, vx2, vy2};
+    pend_vo = a.st_off + (unsigned)(m - m0) * rb;
+  };
+
+  for (int m = m0; m < m1; m += 4) {
+    row(Row<0>{}, m);
+    row(Row<1>{}, m + 1);
+    row(Row<2>{}, m + 2);
+    row(Row<3>{}, m + 3);
   }
+  bstore(pend.r, pend.p, ws, pend_vo, 0);
+  bstore(pend.vx, pend.vy, wu, pend_vo, 0);
 }
 
+AST_DEV float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// The strip of one wave given its workgroup's (band, strip group): fills
+// StripArgs and runs the fast or the boundary variant.  Returns the wave's
+// dt candidate (wave-uniform).
 template <int METHOD, int GEOM, bool WINDOW>
 AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCtx& ex,
                              const float2* __restrict__ s, const float2* __restrict__ u,
@@ -295,12 +416,16 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
   a.cols_in = (band * WOUT - 2 >= 0) && (band * WOUT + 61 < g.N);  // wave-uniform
   a.out_col = (lane >= 2) & (lane < 2 + WOUT) & (a.n < g.N);
   a.xvote = (lane >= 2) & (lane <= 2 + WOUT) & (a.n <= g.N);
-  a.tdV = (float)(tdo / (g.dx * g.dy));
+  a.ax = (float)(-tdo / g.dx);
+  a.ay = (float)(-tdo / g.dy);
   a.tdof = (float)tdo;
   a.dxf = (float)g.dx;
   a.dyf = (float)g.dy;
   a.x = (float)(g.x_min + a.n * g.dx);
   a.ymin = (float)g.y_min;
+  a.row_bytes = (unsigned)g.N * 8u;
+  a.col_off = (unsigned)(a.n < 0 ? 0 : a.n) * 8u;
+  a.st_off = a.out_col ? (unsigned)a.n * 8u : kOffDrop;
   a.ho_x = high_order(g, g.x_min + a.n * g.dx - 0.5 * g.dx);  // west face window
   a.ho_y = high_order(g, g.x_min + a.n * g.dx);               // y faces: cell x
   float UUx = 0.f, UUy = 0.f;
@@ -308,14 +433,20 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
   cw.frozen = c.frozen &&
               frozen_possible(g, band * WOUT - 3, band * WOUT + 62, a.m0 - 2, a.m0 + krows + 1);
   if (a.m0 < g.M) {
-    const bool fast = a.cols_in && a.m0 - 2 >= 0 && a.m1 + 1 < g.M;  // wave-uniform
+    // wave-uniform
+    const bool fast = a.cols_in && a.m0 - 2 >= 0 && a.m1 + 1 < g.M && !cw.frozen &&
+                      ((a.m1 - a.m0) & 3) == 0;
     if (fast)
       run_strip<METHOD, GEOM, WINDOW, true>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
     else
       run_strip<METHOD, GEOM, WINDOW, false>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
   }
+  // only output lanes vote (halo lanes may hold garbage speeds)
+  UUx = wave_max(a.xvote ? UUx : 0.f);
+  UUy = wave_max(a.out_col ? UUy : 0.f);
   // per-face dt = krit*rad/UU (monotone in UU), capped at 1000 (kernel.cu:2530)
-  double tmin = (a.out_col && a.m0 < g.M) ? 1000.0 : INFINITY;
+  const bool any_out = (band * WOUT < g.N) && a.m0 < g.M;  // wave-uniform
+  double tmin = any_out ? 1000.0 : INFINITY;
   if (UUx > 0.f) tmin = fmin(tmin, g.krit * g.dx / (double)UUx);
   if (UUy > 0.f) tmin = fmin(tmin, g.krit * g.dy / (double)UUy);
   return tmin;

@@ -146,7 +146,11 @@ int ast_flux_f32(int method, size_t n, const float* in, float n1, float n2,
  * order).  ast_step_f32 is the fast path: FP32 state, each face evaluated
  * once and shared by its two cells, FP64 kept for the dt min and (exact
  * solver) the p* solve near vacuum; `mlp` (may be NULL) seeds the exact
- * solver's Newton iteration with the learned guess. */
+ * solver's Newton iteration with the learned guess.  The fast path needs
+ * rho > 0 and p > 0 in every cell (a state the step itself keeps) and in
+ * every Dirichlet side state (else AST_ERR_ARG): for such states the
+ * reference's positivity fallback of the reconstructed face values can
+ * never fire, and the fast path does not test for it. */
 int ast_step_f64(const ast_grid_t* g, const double* s, const double* u,
                  double* s2, double* u2, double* T, const double* T_do,
                  double* TT, int method, ast_stats_t* stats, void* stream);
@@ -264,7 +268,8 @@ int ast_sim_get_time(ast_sim_t* sim, double* T, double* T_do, double* TT, int* i
 int ast_sim_get_stats(ast_sim_t* sim, ast_stats_t* out, int reset);
 /* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step. */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
-/* Strip height (rows per wave) of the FP32 step. */
+/* Strip height (rows per wave) of the FP32 step; multiples of 4 take the
+ * fast strip path. */
 int ast_sim_set_tile_rows(ast_sim_t* sim, int rows);
 /* Times the FP32 step for each candidate strip height (NULL = a default
  * list), keeps the fastest; reports it (may be NULL).  State and time are
@@ -307,7 +312,8 @@ void* ast_batch_stream(ast_batch_t* b);
 /* Total cells of all instances (cell-updates per step). */
 long long ast_batch_cells(const ast_batch_t* b);
 
-/* ---- tile configuration of the stand-alone FP32 step entry point -------- */
+/* ---- tile configuration of the stand-alone FP32 step entry point --------
+ * Rows per wave strip (default 8; multiples of 4 take the fast path). */
 int ast_set_tile_rows(int rows);
 int ast_get_tile_rows(void);
 /* Doubles of device memory ast_conservation_sums needs at `out` (the two
