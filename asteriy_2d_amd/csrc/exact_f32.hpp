@@ -11,10 +11,18 @@
 //  * shock + rarefaction: the root of lev (kernel.cu:751-765) on the same
 //    bracket [pI, pII] as devtwo's bisection (kernel.cu:682-749), found by a
 //    bracket-safeguarded Newton iteration to 1e-6 relative;
-//  * FP64 islands: the two-rarefaction closed form near vacuum
-//    ((Udf-Uvk)/(Urz-Uvk) < 0.05, where the FP32 difference cancels) and the
-//    shock + rarefaction solve in strong rarefactions (pI < 1e-4 pII) run in
-//    FP64; counted in ast_stats_t.fp64_faces.
+//  * FP64 islands: near vacuum and in strong rarefactions the p* equation
+//    subtracts two nearly equal numbers, and FP32 would lose the digits of
+//    p* there.  That difference -- and only it: every other term of these
+//    equations is a sum of like-signed terms -- is formed in FP64 from FP64
+//    sound speeds; the solve then runs in FP32 to FP32 accuracy.
+//      two rarefactions, (Udf-Uvk)/(Urz-Uvk) < 0.05: the numerator
+//        Udf - Uvk = (enI - enII) + 2/(g-1) (cI + cII);
+//      shock + rarefaction, pI < 1e-4 pII: the constant D = c2 + du of the
+//        pressure equation fI(p) + c2 q(p) - D = 0 (c2 = 2 cII/(g-1),
+//        du = enI - enII, q = (p/pII)^((g-1)/2g)).
+//    Counted in ast_stats_t.fp64_faces.  (No FP64 pow: the islands keep the
+//    kernel's register budget, and so its occupancy, at the FP32 level.)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -111,12 +119,12 @@ template <> struct FastM<double> {
 };
 
 // Root of lev(p) (left shock + right rarefaction) on [pI, pII]:
-// safeguarded Newton.  R is float or double (FP64 island).
+// safeguarded Newton on fI(p) + c2 q(p) - D with D = c2 + du supplied by the
+// caller (FP64-formed in strong rarefactions).  R is float or double.
 template <typename R>
-AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R rII, R tol,
-                      int& iters, bool& capped) {
+AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R rII, R cI,
+                      R cII, R D, R tol, int& iters, bool& capped) {
   typedef FastM<R> F;
-  const R cI = F::sqrt_(F::div_(g * pI, rI)), cII = F::sqrt_(F::div_(g * pII, rII));
   const R zI = rI * cI, zII = rII * cII, du = enI - enII, c2 = F::div_(R(2), g1) * cII;
   const R ipI = F::div_(R(1), pI), ipII = F::div_(R(1), pII);
   R lo = pI, hi = pII;
@@ -129,7 +137,7 @@ AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R
     const R a = gp * p * ipI + gm;
     const R sa = F::sqrt_(a);
     const R q = F::pow_(p * ipII, gm);
-    const R f = F::div_(p - pI, zI * sa) + c2 * (q - R(1)) - du;
+    const R f = F::div_(p - pI, zI * sa) + c2 * q - D;
     const R df = F::div_((g + R(1)) * p * ipI + (R(3) * g - R(1)), R(4) * g * zI * a * sa) +
                  F::div_(cII * q, g * p);
     if (f < R(0)) lo = p;
@@ -145,12 +153,18 @@ AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R
   return p;
 }
 
+// FP64 sound speed of an FP32 state (for the FP64-formed differences)
+AST_DEV double sound_speed_d(double g, float p, float r) { return sqrt(g * (double)p / (double)r); }
+
 // The exact face flux in the face frame (u normal, v tangential); UU is the
 // largest |wave speed| used for dt.  `guess` (finite, > 0) seeds newton.
-// With e.mlp set the caller guarantees a full wave (EXEC all ones): if any
-// lane of the wave holds a strong two-shock face, the wave evaluates the
-// learned guess for all its lanes on MFMA (a wave-uniform branch) and those
-// lanes start newton from it.
+// LEARNED (compiled only into the kernels that serve a learned guess; its
+// network needs registers the plain solver does not) with e.mlp set: the
+// caller guarantees a full wave (EXEC all ones); if any lane of the wave
+// holds a strong two-shock face, the wave evaluates the learned guess for all
+// its lanes on MFMA (a wave-uniform branch) and those lanes start newton
+// from it.
+template <bool LEARNED>
 AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
                         FaceStats& st, float guess = -1.f, int* info = nullptr) {
   const float g = e.g, g1 = e.g1, g2 = e.g2, gm = e.gm;
@@ -167,7 +181,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   const float Uvk = -e.c2g1 * (cII + cI);
   const float Udf = enI - enII;
   st.exact += 1;
-  if (e.mlp != nullptr) {
+  if (LEARNED && e.mlp != nullptr) {
     const float zI = rI * cI, zII = rII * cII;
     const float pvrs = (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII);
     const bool want = (Udf > Uud) && (pvrs > kStrongShock * pII);
@@ -190,12 +204,12 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
     st.raref += 1;
     const float t = (Udf - Uvk) * frcp(Urz - Uvk);
     if (t < kVacuumIsland) {  // FP64 island: near vacuum
-      const double gd = e.gd, g1d = gd - 1.0, gmd = (g1d / gd) / 2.0;
-      const double cId = sqrt(gd * pI / (double)rI), cIId = sqrt(gd * pII / (double)rII);
-      const double Urzd = -2.0 * cIId / g1d * (1.0 - pow((double)pI / pII, gmd));
-      const double Uvkd = -2.0 * (cIId + cId) / g1d;
-      const double Udfd = (double)enI - (double)enII;
-      p = (float)(pI * pow((Udfd - Uvkd) / (Urzd - Uvkd), 1.0 / gmd));
+      const double gd = e.gd;
+      const double num = ((double)enI - (double)enII) +
+                         2.0 / (gd - 1.0) * (sound_speed_d(gd, pI, rI) + sound_speed_d(gd, pII, rII));
+      // Urz - Uvk = 2/(g-1) (cII q + cI): no cancellation
+      const float den = e.c2g1 * (cII * fpow(pI * frcp(pII), gm) + cI);
+      p = pI * fpow(fmaxf((float)num, 0.f) * frcp(den), e.igm);
       st.fp64 += 1;
     } else {
       p = pI * fpow(t, e.igm);
@@ -203,15 +217,16 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   } else if (Udf <= Uud) {
     pat = 1;
     st.bisect += 1;
+    float D;
     if (pI < kRarefIsland * pII) {  // FP64 island: strong rarefaction
       const double gd = e.gd;
-      p = (float)shock_raref<double>(gd, gd - 1.0, ((gd + 1.0) / gd) / 2.0, ((gd - 1.0) / gd) / 2.0,
-                                     enI, pI, rI, enII, pII, rII, 1e-9, it, capped);
+      D = (float)(2.0 / (gd - 1.0) * sound_speed_d(gd, pII, rII) + ((double)enI - (double)enII));
       st.fp64 += 1;
     } else {
-      p = shock_raref<float>(g, g1, e.gp, gm, enI, pI, rI, enII, pII, rII, kNewtonTol32, it,
-                             capped);
+      D = e.c2g1 * cII + Udf;
     }
+    p = shock_raref<float>(g, g1, e.gp, gm, enI, pI, rI, enII, pII, rII, cI, cII, D, kNewtonTol32,
+                           it, capped);
     st.bisect_it += it;
   } else {
     pat = 2;

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
     else if constexpr (METHOD == AST_HLLC) f = hllc_f32(L, R, ex.g, ex.ig1, UU);
     else {
       int inf[2];
-      f = exact_f32(L, R, ex, UU, st, -1.f, info ? inf : nullptr);
+      f = exact_f32<true>(L, R, ex, UU, st, -1.f, info ? inf : nullptr);
       if (info && valid) {
         info[2 * i] = inf[0];
         info[2 * i + 1] = inf[1];

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   st.clear();
   const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, *T_do,
                                                           wg % nbands, wg / nbands, krows, st);
-  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<NT>(st, redi, stats);
+  if (METHOD >= AST_EXACT && stats != nullptr) block_stats_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, T, nslots);
 }
 
@@ -87,6 +87,11 @@ static int dispatch_geom(const ast_grid_t& g, const Consts& c, const ExactCtx& e
                          const double* T_do, ast_stats_t* stats, int krows, int nslots,
                          hipStream_t st) {
   const bool axi = g.geometry == AST_AXISYMMETRIC;
+  if constexpr (METHOD == kExactLearned) {  // dispatched without a window only
+    (void)window;
+    return axi ? launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
+               : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
+  }
   if (axi) {
     return window ? launch_f32<METHOD, AST_AXISYMMETRIC, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
                   : launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
@@ -122,7 +127,10 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   switch (method) {
     case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
     case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
-    case AST_EXACT: return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+    case AST_EXACT:  // the learned start only where it can run (no window: full waves)
+      if (mlp != nullptr && !window)
+        return dispatch_geom<kExactLearned>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+      return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
     default: return ast_fail(AST_ERR_ARG, "ast_step_f32: unknown method %d", method);
   }
 }

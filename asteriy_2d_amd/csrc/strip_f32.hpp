@@ -155,12 +155,16 @@ static inline int check_positive_bc(const ast_grid_t& g) {
   return AST_OK;
 }
 
+// Kernel-internal method id: the exact solver with the learned newton start
+// compiled in (AST_EXACT kernels leave it out).
+constexpr int kExactLearned = AST_EXACT + 1;
+
 template <int METHOD>
 AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
                         const ExactCtx& ex, FaceStats& st) {
   if constexpr (METHOD == AST_HLL) return hll_f32(L, R, c.g, c.ig1, UU);
   else if constexpr (METHOD == AST_HLLC) return hllc_f32(L, R, c.g, c.ig1, UU);
-  else return exact_f32(L, R, ex, UU, st);
+  else return exact_f32<METHOD == kExactLearned>(L, R, ex, UU, st);
 }
 
 // One face in the x (NORMAL_X) or y direction: rotate to the face frame,
