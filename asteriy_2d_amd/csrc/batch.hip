@@ -35,8 +35,7 @@ struct BatchInst {
 
 template <int METHOD, int GEOM>
 AST_DEV void batch_item(const BatchInst& I, int local, int from, double* red) {
-  FaceStats st;
-  st.clear();
+  NoStats st;
   const double tmin = strip_of_wave<METHOD, GEOM, false>(
       I.g, I.c, I.ex, I.plane[2 * from], I.plane[2 * from + 1], I.plane[2 * (1 - from)],
       I.plane[2 * (1 - from) + 1], I.scal[1], local % I.nbands, local / I.nbands, I.krows, st);

@@ -36,6 +36,8 @@ namespace ast {
 struct ExactCtx {
   float g, g1, g2, gp, gm, igm, ig1, c2g1, g1g2, c2g2;
   double gd;
+  int n;  // 1/gm = 2 gamma/(gamma - 1) when that is an integer in [1, 16] (5 for
+          // gamma = 5/3, 7 for 7/5), else 0
   const unsigned char* mlp;  // learned-guess block (ast_mlp_t) in device memory, or null
 };
 
@@ -54,12 +56,107 @@ static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
   e.g1g2 = (float)(g1 / g2);
   e.c2g2 = (float)(2.0 / g2);
   e.gd = gamma;
+  const double ig = 1.0 / gm, ni = floor(ig + 0.5);
+  // (an FP32 gamma of 5/3 or 7/5 still counts: 1e-6 is its resolution)
+  e.n = (fabs(ig - ni) <= 1e-6 * ig && ni >= 1.0 && ni <= 16.0) ? (int)ni : 0;
   e.mlp = (const unsigned char*)mlp;
   return e;
 }
 
 // x^y for x > 0 via the hardware log2/exp2 (about 1 ulp of FP32 each)
 AST_DEV float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+
+// For an integer n = 1/gm (e.n, wave-uniform): x^n by products and the
+// geometric sum 1 + x + ... + x^(n-1) by Horner's rule, straight-line for
+// n = 5 (gamma 5/3) and n = 7 (gamma 7/5), a loop otherwise.
+AST_DEV float ipow(float x, int n) {
+  if (n == 5) {
+    const float x2 = x * x;
+    return x2 * x2 * x;
+  }
+  if (n == 7) {
+    const float x3 = x * x * x;
+    return x3 * x3 * x;
+  }
+  float p = x;
+  for (int k = 1; k < n; k++) p *= x;
+  return p;
+}
+AST_DEV float geom_sum(float x, int n) {
+  if (n == 5) return fmaf(fmaf(fmaf(fmaf(x, 1.f, 1.f), x, 1.f), x, 1.f), x, 1.f);
+  if (n == 7)
+    return fmaf(fmaf(fmaf(fmaf(fmaf(fmaf(x, 1.f, 1.f), x, 1.f), x, 1.f), x, 1.f), x, 1.f), x, 1.f);
+  float s = 1.f;
+  for (int k = 1; k < n; k++) s = fmaf(s, x, 1.f);
+  return s;
+}
+// x^(1/gm): exact products for an integer 1/gm, else exp2/log2
+AST_DEV float pow_igm(const ExactCtx& e, float x) { return e.n ? ipow(x, e.n) : fpow(x, e.igm); }
+
+// What the exact solver did on one face (pat: -1 vacuum, 0 two
+// rarefactions, 1 shock + rarefaction, 2 two shocks; kNoFace: not an exact
+// face, e.g. a low-order face of the window).
+constexpr int kNoFace = -3;
+struct FaceInfo {
+  int pat, it;
+  bool capped, island, learned;
+};
+
+AST_DEV uint32_t wcount(bool c) {
+  return (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(c));
+}
+
+// The counter type of kernels that count nothing (counting costs the exact
+// kernels a third of their time, so it is compiled only into the variants
+// that are asked for counters).
+struct NoStats {
+  AST_DEV void clear() {}
+  AST_DEV void add(const struct FaceInfo&) {}
+};
+
+// Solver counters of the FP32 exact solver.  Face counts are per wave
+// (ballot + popcount: wave-uniform, kept in scalar registers), the iteration
+// sums per lane.  Per-lane face counters would hold ten vector registers for
+// the whole strip and cost the kernel a wave of occupancy.  add() runs in
+// converged code (every lane of the wave active), after the face calls.
+struct StatsF32 {
+  uint32_t exact, newton, bisect, raref, vacuum, capped, fp64, mlp;  // per wave
+  uint32_t newton_it, bisect_it;                                     // per lane
+  AST_DEV void clear() {
+    exact = newton = bisect = raref = vacuum = capped = fp64 = mlp = 0;
+    newton_it = bisect_it = 0;
+  }
+  AST_DEV void add(const FaceInfo& f) {
+    exact += wcount(f.pat != kNoFace);
+    raref += wcount(f.pat == 0);
+    vacuum += wcount(f.pat == -1);
+    bisect += wcount(f.pat == 1);
+    newton += wcount(f.pat == 2);
+    bisect_it += f.pat == 1 ? f.it : 0;
+    newton_it += f.pat == 2 ? f.it : 0;
+    capped += wcount(f.capped);
+    fp64 += wcount(f.island);
+    mlp += wcount(f.learned);
+  }
+};
+
+// Adds a block's StatsF32 to *stats (ast_stats_t order); full workgroup.
+template <int NT>
+AST_DEV void block_stats_f32_to(const StatsF32& st, uint32_t* red, ast_stats_t* stats) {
+  constexpr int NW = NT / 64;
+  const uint32_t v[10] = {st.exact, st.newton, wave_sum(st.newton_it), st.bisect,
+                          wave_sum(st.bisect_it), st.raref, st.vacuum, st.capped, st.fp64,
+                          st.mlp};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int i = 0; i < 10; i++) red[wave * 10 + i] = v[i];
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    unsigned long long t = 0;
+    for (int w = 0; w < NW; w++) t += red[w * 10 + threadIdx.x];
+    if (t) atomicAdd(reinterpret_cast<unsigned long long*>(stats) + threadIdx.x, t);
+  }
+}
 
 constexpr float kNewtonTol32 = 1e-6f;
 // a two-shock face is served by the learned guess when the PVRS start is
@@ -164,9 +261,19 @@ AST_DEV double sound_speed_d(double g, float p, float r) { return sqrt(g * (doub
 // holds a strong two-shock face, the wave evaluates the learned guess for all
 // its lanes on MFMA (a wave-uniform branch) and those lanes start newton
 // from it.
+// The exact face flux in the face frame (u normal, v tangential), kernel.cu:
+// 128-570 in FP32 (see the header); UU is the largest |wave speed| used for
+// dt; fi reports the wave pattern, iterations and flags.  `guess` (finite,
+// > 0) seeds newton.
+// LEARNED (compiled only into the kernels that serve a learned guess; its
+// network needs registers the plain solver does not) with e.mlp set: the
+// caller guarantees a full wave (EXEC all ones); if any lane of the wave
+// holds a strong two-shock face, the wave evaluates the learned guess for
+// all its lanes on MFMA (a wave-uniform branch) and those lanes start
+// newton from it.
 template <bool LEARNED>
 AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
-                        FaceStats& st, float guess = -1.f, int* info = nullptr) {
+                        FaceInfo& fi, float guess = -1.f) {
   const float g = e.g, g1 = e.g1, g2 = e.g2, gm = e.gm;
   // swap sides so that pI <= pII, kernel.cu:156-184
   const bool sw = L.p > R.p;
@@ -175,74 +282,64 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   const float pI = sw ? R.p : L.p, pII = sw ? L.p : R.p;
   const float rI = sw ? R.r : L.r, rII = sw ? L.r : R.r;
   const float cI = fsqrt(g * pI * frcp(rI)), cII = fsqrt(g * pII * frcp(rII));
+  const float q = fpow(pI * frcp(pII), gm);  // (pI/pII)^gm
   const float a = fsqrt(rI * (g2 * pII + g1 * pI) * 0.5f);
   const float Uud = (pII - pI) * frcp(a);
-  const float Urz = -e.c2g1 * cII * (1.f - fpow(pI * frcp(pII), gm));
+  const float Urz = -e.c2g1 * cII * (1.f - q);
   const float Uvk = -e.c2g1 * (cII + cI);
   const float Udf = enI - enII;
-  st.exact += 1;
   if (LEARNED && e.mlp != nullptr) {
     const float zI = rI * cI, zII = rII * cII;
     const float pvrs = (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII);
     const bool want = (Udf > Uud) && (pvrs > kStrongShock * pII);
     if (__builtin_amdgcn_ballot_w64(want) != 0) {  // wave-uniform
-      float f[mlp::IN];
-      const float lp = mlp::features<float>(rI, pI, enI, rII, pII, enII, g, f);
-      const float y = mlp::forward_wave_bf16(e.mlp, f);
+      float fe[mlp::IN];
+      const float lp = mlp::features<float>(rI, pI, enI, rII, pII, enII, g, fe);
+      const float y = mlp::forward_wave_bf16(e.mlp, fe);
       if (want) guess = __expf(y + lp);
     }
   }
   int pat;
-  float p = 0.f;
+  float p = 0.f, t = 1.f;
   int it = 0;
-  bool capped = false;
+  bool capped = false, island = false, learned = false;
   if (Udf < Uvk) {
     pat = -1;
-    st.vacuum += 1;
   } else if (Udf <= Urz) {
+    // two rarefactions, Urz - Uvk = 2/(g-1) (cII q + cI) (no cancellation)
     pat = 0;
-    st.raref += 1;
-    const float t = (Udf - Uvk) * frcp(Urz - Uvk);
+    const float den = e.c2g1 * (cII * q + cI);
+    t = (Udf - Uvk) * frcp(den);
     if (t < kVacuumIsland) {  // FP64 island: near vacuum
       const double gd = e.gd;
       const double num = ((double)enI - (double)enII) +
                          2.0 / (gd - 1.0) * (sound_speed_d(gd, pI, rI) + sound_speed_d(gd, pII, rII));
-      // Urz - Uvk = 2/(g-1) (cII q + cI): no cancellation
-      const float den = e.c2g1 * (cII * fpow(pI * frcp(pII), gm) + cI);
-      p = pI * fpow(fmaxf((float)num, 0.f) * frcp(den), e.igm);
-      st.fp64 += 1;
-    } else {
-      p = pI * fpow(t, e.igm);
+      t = fmaxf((float)num, 0.f) * frcp(den);
+      island = true;
     }
+    p = pI * pow_igm(e, t);
   } else if (Udf <= Uud) {
     pat = 1;
-    st.bisect += 1;
     float D;
     if (pI < kRarefIsland * pII) {  // FP64 island: strong rarefaction
       const double gd = e.gd;
       D = (float)(2.0 / (gd - 1.0) * sound_speed_d(gd, pII, rII) + ((double)enI - (double)enII));
-      st.fp64 += 1;
+      island = true;
     } else {
       D = e.c2g1 * cII + Udf;
     }
     p = shock_raref<float>(g, g1, e.gp, gm, enI, pI, rI, enII, pII, rII, cI, cII, D, kNewtonTol32,
                            it, capped);
-    st.bisect_it += it;
   } else {
     pat = 2;
-    st.newton += 1;
-    const bool ug = guess > 0.f && guess < 3.0e38f;
-    st.mlp += ug ? 1 : 0;
-    p = newton_f32(e, enI, pI, rI, cI, enII, pII, rII, cII, guess, ug, it, capped);
-    st.newton_it += it;
+    learned = guess > 0.f && guess < 3.0e38f;
+    p = newton_f32(e, enI, pI, rI, cI, enII, pII, rII, cII, guess, learned, it, capped);
   }
-  st.capped += capped ? 1 : 0;
-  if (info) {
-    const bool island = (pat == 0 && (Udf - Uvk) * frcp(Urz - Uvk) < kVacuumIsland) ||
-                        (pat == 1 && pI < kRarefIsland * pII);
-    info[0] = it;
-    info[1] = (pat + 1) | (capped ? 8 : 0) | (island ? 16 : 0);
-  }
+  fi.pat = pat;
+  fi.it = it;
+  fi.capped = capped;
+  fi.island = island;
+  fi.learned = learned;
 
   // sample the solution at w = 0, kernel.cu:234-544
   float en, pp, r, te;
@@ -272,14 +369,21 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
       en = u; pp = p; r = g * p * frcp(ce * ce); te = teII;
     } else if (0.f < dII) {
       const float ce = -e.g1g2 * enII + e.c2g2 * cII;
-      en = -ce; pp = pII * fpow(ce * frcp(cII), e.igm); r = g * pp * frcp(ce * ce); te = teII;
+      en = -ce; pp = pII * pow_igm(e, ce * frcp(cII)); r = g * pp * frcp(ce * ce); te = teII;
     } else { en = enII; pp = pII; r = rII; te = teII; }
   } else if (pat == 0) {
+    // rarefaction impedances aK = gm rK cK (1 - x)/(1 - x^gm), x = p/pK
+    // (kernel.cu:369-388), with x^gm = t (side I) and t q (side II)
     float aI, aII;
-    if (fabsf(p - pI) < 1e-6f * pI) aI = rI * cI;
-    else aI = gm * rI * cI * (1.f - p * frcp(pI)) * frcp(1.f - fpow(p * frcp(pI), gm));
-    if (fabsf(p - pII) < 1e-6f * pII) aII = rII * cII;
-    else aII = gm * rII * cII * (1.f - p * frcp(pII)) * frcp(1.f - fpow(p * frcp(pII), gm));
+    if (e.n) {
+      aI = gm * rI * cI * geom_sum(t, e.n);
+      aII = gm * rII * cII * geom_sum(t * q, e.n);
+    } else {
+      if (fabsf(p - pI) < 1e-6f * pI) aI = rI * cI;
+      else aI = gm * rI * cI * (1.f - p * frcp(pI)) * frcp(1.f - t);
+      if (fabsf(p - pII) < 1e-6f * pII) aII = rII * cII;
+      else aII = gm * rII * cII * (1.f - p * frcp(pII)) * frcp(1.f - t * q);
+    }
     const float u = (aI * enI + aII * enII + pI - pII) * frcp(aI + aII);
     const float dI = enI - cI, ddI = u - cI - g1 * (enI - u) * 0.5f;
     const float dII = enII + cII, ddII = u + cII - g1 * (enII - u) * 0.5f;
@@ -287,7 +391,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
     if (0.f <= dI) { en = enI; pp = pI; r = rI; te = teI; }
     else if (0.f < ddI) {
       const float ce = e.g1g2 * enI + e.c2g2 * cI;
-      en = ce; pp = pI * fpow(ce * frcp(cI), e.igm); r = g * pp * frcp(ce * ce); te = teI;
+      en = ce; pp = pI * pow_igm(e, ce * frcp(cI)); r = g * pp * frcp(ce * ce); te = teI;
     } else if (0.f <= u) {
       const float ce = cI + g1 * 0.5f * (enI - u);
       en = u; pp = p; r = g * p * frcp(ce * ce); te = teI;
@@ -296,7 +400,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
       en = u; pp = p; r = g * p * frcp(ce * ce); te = teII;
     } else if (0.f < dII) {
       const float ce = -e.g1g2 * enII + e.c2g2 * cII;
-      en = -ce; pp = pII * fpow(ce * frcp(cII), e.igm); r = g * pp * frcp(ce * ce); te = teII;
+      en = -ce; pp = pII * pow_igm(e, ce * frcp(cII)); r = g * pp * frcp(ce * ce); te = teII;
     } else { en = enII; pp = pII; r = rII; te = teII; }
   } else {  // vacuum, kernel.cu:480-544
     const float dI = enI - cI, ddI = enI + e.c2g1 * cI;
@@ -305,11 +409,11 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
     if (0.f <= dI) { en = enI; pp = pI; r = rI; te = teI; }
     else if (0.f < ddI) {
       const float ce = e.g1g2 * enI + e.c2g2 * cI;
-      en = ce; pp = pI * fpow(ce * frcp(cI), e.igm); r = g * pp * frcp(ce * ce); te = teI;
+      en = ce; pp = pI * pow_igm(e, ce * frcp(cI)); r = g * pp * frcp(ce * ce); te = teI;
     } else if (0.f <= ddII) { en = 0.f; pp = 0.f; r = 0.f; te = 0.f; }
     else if (0.f < dII) {
       const float ce = -e.g1g2 * enII + e.c2g2 * cII;
-      en = -ce; pp = pII * fpow(ce * frcp(cII), e.igm); r = g * pp * frcp(ce * ce); te = teII;
+      en = -ce; pp = pII * pow_igm(e, ce * frcp(cII)); r = g * pp * frcp(ce * ce); te = teII;
     } else { en = enII; pp = pII; r = rII; te = teII; }
   }
   if (!(UU > 1e-7f)) UU = 1e-7f;  // eps8 floor of the reference's dt, kernel.cu:246-253

@@ -2,6 +2,8 @@
 // HLL, HLLC and the exact solver (FP64 islands) of the fast path applied to
 // n independent faces with a common normal (n1, n2).  52 B/face of
 // algorithmic traffic (8 words in, 5 out).
+#include <type_traits>
+
 #include "common.hpp"
 #include "exact_f32.hpp"
 #include "fast_flux.hpp"
@@ -11,13 +13,14 @@ namespace f32 {
 
 constexpr int FNT = 256;
 
-template <int METHOD>
+// STATS: the variant that counts the exact solver's work into *stats
+template <int METHOD, bool STATS>
 __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __restrict__ in,
                                                    float n1, float n2, float krad,
                                                    ExactCtx ex, float* __restrict__ out,
                                                    ast_stats_t* stats, int* __restrict__ info) {
   __shared__ uint32_t redi[(FNT / 64) * 10];
-  FaceStats st;
+  typename std::conditional<STATS, StatsF32, NoStats>::type st;
   st.clear();
   // whole 64-face tiles per wave: every lane runs every iteration (EXEC full,
   // as the wave-level learned guess needs); lanes past n compute on a clamped
@@ -40,11 +43,13 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
     if constexpr (METHOD == AST_HLL) f = hll_f32(L, R, ex.g, ex.ig1, UU);
     else if constexpr (METHOD == AST_HLLC) f = hllc_f32(L, R, ex.g, ex.ig1, UU);
     else {
-      int inf[2];
-      f = exact_f32<true>(L, R, ex, UU, st, -1.f, info ? inf : nullptr);
+      FaceInfo fi;
+      f = exact_f32<true>(L, R, ex, UU, fi);
+      if (!valid) fi = FaceInfo{kNoFace, 0, false, false, false};
+      st.add(fi);
       if (info && valid) {
-        info[2 * i] = inf[0];
-        info[2 * i + 1] = inf[1];
+        info[2 * i] = fi.it;
+        info[2 * i + 1] = (fi.pat + 1) | (fi.capped ? 8 : 0) | (fi.island ? 16 : 0);
       }
     }
     if (valid) {
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
       out[4 * n + i] = krad * frcp(UU);
     }
   }
-  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<FNT>(st, redi, stats);
+  if constexpr (STATS) block_stats_f32_to<FNT>(st, redi, stats);
 }
 
 template <int METHOD>
@@ -63,8 +68,12 @@ static int launch(size_t n, const float* in, float n1, float n2, float krad, con
                   float* out, ast_stats_t* stats, int* info, hipStream_t stream) {
   size_t blocks = (n + FNT - 1) / FNT;
   if (blocks > 256 * 64) blocks = 256 * 64;
-  hipLaunchKernelGGL(flux_kernel<METHOD>, dim3((unsigned)blocks), dim3(FNT), 0, stream, n, in, n1,
-                     n2, krad, ex, out, stats, info);
+  if (METHOD == AST_EXACT && stats != nullptr)
+    hipLaunchKernelGGL((flux_kernel<METHOD, true>), dim3((unsigned)blocks), dim3(FNT), 0, stream, n,
+                       in, n1, n2, krad, ex, out, stats, info);
+  else
+    hipLaunchKernelGGL((flux_kernel<METHOD, false>), dim3((unsigned)blocks), dim3(FNT), 0, stream, n,
+                       in, n1, n2, krad, ex, out, stats, info);
   AST_LAUNCH_CHECK("ast_flux_f32 launch");
   return AST_OK;
 }

@@ -44,6 +44,7 @@ struct ast_sim {
   double* scal;    // time block: T, T_do, TT (+ int i in the 4th word), then
                    // the FP32 step's dt slots (time_slots.h)
   ast_stats_t* stats;
+  bool collect;  // count the exact solver's work into stats (counting kernels)
   const ast_mlp_t* mlp;
   hipStream_t stream;
   hipGraph_t graph;
@@ -63,11 +64,11 @@ static int launch_step(ast_sim_t* S, int from) {
   int rc;
   if (S->precision == 0)
     rc = ast_step_f64(&S->g, (const double*)s, (const double*)u, (double*)s2, (double*)u2, T,
-                      T + 1, T + 2, S->method, S->stats, S->stream);
+                      T + 1, T + 2, S->method, S->collect ? S->stats : nullptr, S->stream);
   else  // dt spread over the slots of the time block, folded by the time advance
     rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
-                           T + 1, S->method, S->mlp, S->stats, S->krows, ast::kMaxSlots,
-                           S->stream);
+                           T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
+                           ast::kMaxSlots, S->stream);
   if (rc != AST_OK) return rc;
   if (S->precision == 1) return ast_funk_time_slots(T, ast::kMaxSlots, S->stream);
   return ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
@@ -217,6 +218,16 @@ extern "C" int ast_sim_get_stats(ast_sim_t* S, ast_stats_t* out, int reset) {
   CK(hipStreamSynchronize(S->stream));
   CK(hipMemcpy(out, S->stats, sizeof *out, hipMemcpyDeviceToHost));
   if (reset) CK(hipMemset(S->stats, 0, sizeof(ast_stats_t)));
+  return AST_OK;
+}
+
+extern "C" int ast_sim_collect_stats(ast_sim_t* S, int on) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_collect_stats: null sim");
+  CK(hipStreamSynchronize(S->stream));
+  if ((on != 0) != S->collect) {
+    S->collect = on != 0;
+    drop_graph(S);  // the captured step kernels change
+  }
   return AST_OK;
 }
 

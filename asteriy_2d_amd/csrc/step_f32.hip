@@ -26,12 +26,15 @@
 //  * the exact solver runs in FP32 with its p* solve promoted to FP64 near
 //    vacuum and in strong rarefactions, optionally seeded by the learned
 //    guess (exact_f32.hpp).
+#include <type_traits>
+
 #include "strip_f32.hpp"
 
 namespace ast {
 namespace f32 {
 
-template <int METHOD, int GEOM, bool WINDOW>
+// STATS: the variant that counts the exact solver's work into *stats
+template <int METHOD, int GEOM, bool WINDOW, bool STATS>
 __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactCtx ex,
                                                   const float2* __restrict__ s,
                                                   const float2* __restrict__ u,
@@ -43,11 +46,11 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
   const int wg = xcd_tile(blockIdx.x, nwg);
-  FaceStats st;
+  typename std::conditional<STATS, StatsF32, NoStats>::type st;
   st.clear();
   const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, *T_do,
                                                           wg % nbands, wg / nbands, krows, st);
-  if (METHOD >= AST_EXACT && stats != nullptr) block_stats_to<NT>(st, redi, stats);
+  if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, T, nslots);
 }
 
@@ -75,8 +78,16 @@ static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, 
   const int nsg = (nstrips + WPB - 1) / WPB;
   const long long nwg = (long long)nbands * nsg;
   if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
-  hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW>), dim3((unsigned)nwg), dim3(NT), 0,
-                     stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg, nslots);
+  if (METHOD >= AST_EXACT && stats != nullptr) {
+    if constexpr (METHOD >= AST_EXACT)
+      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3((unsigned)nwg), dim3(NT), 0,
+                         stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg,
+                         nslots);
+  } else {
+    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false>), dim3((unsigned)nwg), dim3(NT), 0,
+                       stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg,
+                       nslots);
+  }
   AST_LAUNCH_CHECK("ast_step_f32 launch");
   return AST_OK;
 }

@@ -180,7 +180,9 @@ AST_DEV double cell(const ast_grid_t& g, const Phys<double>& ph, const Tile& L, 
   return tmin;
 }
 
-template <int METHOD, int GEOM>
+// STATS: the variant that counts the exact solver's work into *stats (the
+// per-face counters are dead code, and removed, in the other)
+template <int METHOD, int GEOM, bool STATS>
 __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, const double2* __restrict__ s,
                                                   const double2* __restrict__ u,
                                                   double2* __restrict__ s2,
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, const double2* _
     u2[k] = make_double2(uo.x, uo.y);
   }
   block_min_to<NT>(tmin, red, T);
-  if (METHOD == AST_EXACT && stats != nullptr) block_stats_to<NT>(st, redi, stats);
+  if constexpr (STATS) block_stats_to<NT>(st, redi, stats);
 }
 
 template <int METHOD, int GEOM>
@@ -227,8 +229,12 @@ static int launch(const ast_grid_t& g, const double2* s, const double2* u, doubl
                   hipStream_t stream) {
   const int tiles_x = (g.N + TX - 1) / TX, tiles_y = (g.M + TY - 1) / TY;
   const int ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL((step_kernel<METHOD, GEOM>), dim3(ntiles), dim3(NT), 0, stream, g, s, u, s2,
-                     u2, T, T_do, stats, tiles_x, ntiles);
+  if (METHOD == AST_EXACT && stats != nullptr)
+    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, true>), dim3(ntiles), dim3(NT), 0, stream, g, s,
+                       u, s2, u2, T, T_do, stats, tiles_x, ntiles);
+  else
+    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, false>), dim3(ntiles), dim3(NT), 0, stream, g, s,
+                       u, s2, u2, T, T_do, stats, tiles_x, ntiles);
   AST_LAUNCH_CHECK("ast_step_f64 launch");
   return AST_OK;
 }

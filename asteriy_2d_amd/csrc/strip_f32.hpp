@@ -161,35 +161,45 @@ constexpr int kExactLearned = AST_EXACT + 1;
 
 template <int METHOD>
 AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
-                        const ExactCtx& ex, FaceStats& st) {
+                        const ExactCtx& ex, FaceInfo& fi) {
   if constexpr (METHOD == AST_HLL) return hll_f32(L, R, c.g, c.ig1, UU);
   else if constexpr (METHOD == AST_HLLC) return hllc_f32(L, R, c.g, c.ig1, UU);
-  else return exact_f32<METHOD == kExactLearned>(L, R, ex, UU, st);
+  else return exact_f32<METHOD == kExactLearned>(L, R, ex, UU, fi);
 }
 
 // One face in the x (NORMAL_X) or y direction: rotate to the face frame,
 // flux, and back.  With WINDOW, faces outside the high-order window use
 // HLL on the cell states (kernel.cu:2812, 2838-2839, 2856).
-template <int METHOD, bool WINDOW, bool NORMAL_X>
+// The exact solver's FaceInfo goes to st.add() after the call, in converged
+// code (faces diverge inside with a window).
+template <int METHOD, bool WINDOW, bool NORMAL_X, class ST>
 AST_DEV FluxF face(const Q& l, const Q& r, const Q& lc, const Q& rc, bool ho, const Consts& c,
-                   float& UU, const ExactCtx& ex, FaceStats& st) {
+                   float& UU, const ExactCtx& ex, ST& st) {
   PrimF L, R;
+  FaceInfo fi;
+  fi.pat = kNoFace;
+  fi.it = 0;
+  fi.capped = fi.island = fi.learned = false;
+  FluxF f;
   if constexpr (WINDOW) {  // faces diverge here: no wave-level learned guess
     if (!ho) {
       L = NORMAL_X ? PrimF{lc.r, lc.p, lc.vx, lc.vy} : PrimF{lc.r, lc.p, lc.vy, lc.vx};
       R = NORMAL_X ? PrimF{rc.r, rc.p, rc.vx, rc.vy} : PrimF{rc.r, rc.p, rc.vy, rc.vx};
-      return hll_f32(L, R, c.g, c.ig1, UU);
+      f = hll_f32(L, R, c.g, c.ig1, UU);
+    } else {
+      ExactCtx e2 = ex;
+      e2.mlp = nullptr;
+      L = NORMAL_X ? PrimF{l.r, l.p, l.vx, l.vy} : PrimF{l.r, l.p, l.vy, l.vx};
+      R = NORMAL_X ? PrimF{r.r, r.p, r.vx, r.vy} : PrimF{r.r, r.p, r.vy, r.vx};
+      f = face_flux<METHOD>(L, R, c, UU, e2, fi);
     }
-    ExactCtx e2 = ex;
-    e2.mlp = nullptr;
-    L = NORMAL_X ? PrimF{l.r, l.p, l.vx, l.vy} : PrimF{l.r, l.p, l.vy, l.vx};
-    R = NORMAL_X ? PrimF{r.r, r.p, r.vx, r.vy} : PrimF{r.r, r.p, r.vy, r.vx};
-    return face_flux<METHOD>(L, R, c, UU, e2, st);
   } else {
     L = NORMAL_X ? PrimF{l.r, l.p, l.vx, l.vy} : PrimF{l.r, l.p, l.vy, l.vx};
     R = NORMAL_X ? PrimF{r.r, r.p, r.vx, r.vy} : PrimF{r.r, r.p, r.vy, r.vx};
-    return face_flux<METHOD>(L, R, c, UU, ex, st);
+    f = face_flux<METHOD>(L, R, c, UU, ex, fi);
   }
+  if constexpr (METHOD >= AST_EXACT) st.add(fi);
+  return f;
 }
 
 AST_DEV bool frozen_at(const Consts& c, float x, float y) {
@@ -238,11 +248,11 @@ struct Row {
 // stores of the lanes that own no output go to an offset beyond the store
 // range and are dropped by the range check, so every row issues the same
 // memory operations and the load of row m + 3 never waits for a store.
-template <int METHOD, int GEOM, bool WINDOW, bool FAST>
+template <int METHOD, int GEOM, bool WINDOW, bool FAST, class ST>
 AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, const ExactCtx& ex,
                        const float2* __restrict__ s, const float2* __restrict__ u,
                        float2* __restrict__ s2, float2* __restrict__ u2, const StripArgs& a,
-                       float& UUx, float& UUy, FaceStats& st) {
+                       float& UUx, float& UUy, ST& st) {
   const int n = a.n, m0 = a.m0, m1 = a.m1, N = g.N;
   const unsigned rb = a.row_bytes;
   // loads: FAST strips read rows m0-2 .. m1+1 through one resource per plane
@@ -405,11 +415,11 @@ AST_DEV float wave_max(float v) {
 // The strip of one wave given its workgroup's (band, strip group): fills
 // StripArgs and runs the fast or the boundary variant.  Returns the wave's
 // dt candidate (wave-uniform).
-template <int METHOD, int GEOM, bool WINDOW>
+template <int METHOD, int GEOM, bool WINDOW, class ST>
 AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCtx& ex,
                              const float2* __restrict__ s, const float2* __restrict__ u,
                              float2* __restrict__ s2, float2* __restrict__ u2, double tdo,
-                             int band, int sgrp, int krows, FaceStats& st) {
+                             int band, int sgrp, int krows, ST& st) {
   // wave index made explicitly uniform: rows and the loop stay scalar
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;

@@ -26,6 +26,7 @@ SIM_SIGNATURES = {
     "ast_sim_set_time": (_c_int, [_vp, _c_double, _c_double, _c_double, _c_int]),
     "ast_sim_get_time": (_c_int, [_vp, _dp, _dp, _dp, ctypes.POINTER(_c_int)]),
     "ast_sim_get_stats": (_c_int, [_vp, ctypes.POINTER(CStats), _c_int]),
+    "ast_sim_collect_stats": (_c_int, [_vp, _c_int]),
     "ast_sim_set_mlp": (_c_int, [_vp, _vp]),
     "ast_sim_set_tile_rows": (_c_int, [_vp, _c_int]),
     "ast_sim_stream": (_vp, [_vp]),
@@ -109,6 +110,10 @@ class Simulation:
         native.call("ast_sim_get_time", self.h, ctypes.byref(T), ctypes.byref(Td),
                     ctypes.byref(TT), ctypes.byref(i))
         return {"T": T.value, "T_do": Td.value, "TT": TT.value, "i": i.value}
+
+    def collect_stats(self, on: bool = True):
+        """Count the exact solver's work (slower kernel variants) from now on."""
+        native.call("ast_sim_collect_stats", self.h, 1 if on else 0)
 
     def stats(self, reset=False):
         st = CStats()

@@ -106,7 +106,9 @@ def _errors(x, ref):
             "linf_p": _linf(x[0][..., 1], ref[0][..., 1])}
 
 
-def _run_sim(grid, method, precision, s, u, steps, mlp=None, autotune=False):
+def _run_sim(grid, method, precision, s, u, steps, mlp=None, autotune=False, stats=False):
+    """Timed run (no solver counting); with stats, the same steps again from the
+    same state with the counting kernels, for the solver counters."""
     from asteriy_2d_amd.sim import Simulation
     sim = Simulation(grid, method, precision, s, u)
     if mlp is not None:
@@ -117,9 +119,19 @@ def _run_sim(grid, method, precision, s, u, steps, mlp=None, autotune=False):
     ms = _timed(sim, steps, sim.torch_stream())
     st = sim.state()
     t = sim.time()
-    stats = sim.stats()
     sim.close()
-    return ms, st, t, stats, rows
+    counters = None
+    if stats:
+        sim = Simulation(grid, method, precision, s, u)
+        if mlp is not None:
+            sim.set_mlp(mlp)
+        if rows:
+            sim.set_tile_rows(rows)
+        sim.collect_stats(True)
+        sim.run(steps)
+        counters = sim.stats()
+        sim.close()
+    return ms, st, t, counters, rows
 
 
 def _cpu_baseline(grid, s, u, method, budget_s: float):
@@ -251,13 +263,14 @@ def main(argv=None):
         # -- exact solver at 2048x1024: FP32 (+FP64 islands) vs FP64 ----------
         Ke = 200
         pe = problems.wind(N_GRID, M_GRID, method=EXACT, steps=Ke)
-        me32, e32, _, sx, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, autotune=True)
-        me64, e64, _, s64x, _ = _run_sim(g, EXACT, "f64", pe.s, pe.u, Ke)
+        me32, e32, _, sx, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, autotune=True, stats=True)
+        me64, e64, _, s64x, _ = _run_sim(g, EXACT, "f64", pe.s, pe.u, Ke, stats=True)
         extra["exact_2048x1024"] = {
             "steps": Ke,
             "fp32_gcells": round(cells * Ke / me32 / 1e6, 3),
             "fp64_gcells": round(cells * Ke / me64 / 1e6, 3),
             "speedup": round(me64 / me32, 3),
+            "fp32_exact_vs_fp64_hllc": round(ms64 / K * Ke / me32, 3),
             "error_vs_fp64": _errors(e32, e64),
             "newton_iters_per_face_fp32": round(sx["newton_iters"] / max(1, sx["newton_faces"]), 3),
             "newton_iters_per_face_fp64": round(s64x["newton_iters"] / max(1, s64x["newton_faces"]), 3),
@@ -272,8 +285,9 @@ def main(argv=None):
         pc = problems.wind(4096, 2048, method=EXACT)
         gc = pc.grid
         Kc = 100
-        mc_m, c_m, _, sc_m, _ = _run_sim(gc, EXACT, "f32", pc.s, pc.u, Kc, mlp=mdl, autotune=True)
-        mc_0, _, _, sc_0, _ = _run_sim(gc, EXACT, "f32", pc.s, pc.u, Kc, autotune=True)
+        mc_m, c_m, _, sc_m, _ = _run_sim(gc, EXACT, "f32", pc.s, pc.u, Kc, mlp=mdl, autotune=True,
+                                         stats=True)
+        mc_0, _, _, sc_0, _ = _run_sim(gc, EXACT, "f32", pc.s, pc.u, Kc, autotune=True, stats=True)
         mc_64, c_64, _, _, _ = _run_sim(gc, EXACT, "f64", pc.s, pc.u, Kc)
         extra["exact_learned_4096x2048"] = {
             "steps": Kc,

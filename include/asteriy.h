@@ -264,8 +264,11 @@ int ast_sim_get_state(ast_sim_t* sim, double* s, double* u);
 int ast_sim_device_state(ast_sim_t* sim, void** s, void** u);
 int ast_sim_set_time(ast_sim_t* sim, double T, double T_do, double TT, int i);
 int ast_sim_get_time(ast_sim_t* sim, double* T, double* T_do, double* TT, int* i);
-/* Exact-solver counters accumulated by the simulation (reset on request). */
+/* Exact-solver counters accumulated by the simulation (reset on request),
+ * while counting is on (ast_sim_collect_stats; default off: the counting
+ * variants of the exact-solver kernels run markedly slower). */
 int ast_sim_get_stats(ast_sim_t* sim, ast_stats_t* out, int reset);
+int ast_sim_collect_stats(ast_sim_t* sim, int on);
 /* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step. */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
 /* Strip height (rows per wave) of the FP32 step; multiples of 4 take the
