@@ -173,8 +173,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
-    ap.add_argument("--profile", choices=["f32", "f64"], default=None,
-                    help="only launch the headline step kernel --steps times (eager, for rocprofv3)")
+    ap.add_argument("--profile", choices=["f32", "f64", "exact32", "exact64"], default=None,
+                    help="only launch one step kernel --steps times (eager, for rocprofv3): the "
+                         "HLLC step in FP32 / FP64, or --steps exact-solver steps of the wind "
+                         "workload from its initial state")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer --steps times (counter calibration)")
     a = ap.parse_args(argv)
@@ -204,6 +206,13 @@ def main(argv=None):
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copied {K} x {src.numel() * 4} bytes")
+        return 0
+    if a.profile in ("exact32", "exact64"):
+        sim = Simulation(g, EXACT, "f32" if a.profile == "exact32" else "f64", p.s, p.u)
+        sim.run(K, graph=False)
+        sim.sync()
+        log(f"{a.profile}: {K} steps")
+        sim.close()
         return 0
     if a.profile:
         rows = 8

@@ -189,6 +189,30 @@ def test_sim_graph_equals_eager(dev, precision):
     assert ta["TT"] == TT and ta["i"] == 41
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_sim_counting_kernels_change_nothing(dev, precision):
+    """The exact-solver kernels compiled with the solver counters step
+    bit for bit like the ones without; counters accumulate only while
+    counting is on, and count every face once (FP32: the strip seams
+    evaluate some faces twice, FP64: each cell evaluates its four faces)."""
+    from asteriy_2d_amd.sim import Simulation
+    p = problems.wind(256, 128)
+    res = []
+    for collect in (False, True):
+        with Simulation(p.grid, G.EXACT, precision, p.s, p.u) as sim:
+            sim.collect_stats(collect)
+            sim.run(9)
+            res.append((sim.state(), sim.time(), sim.stats()))
+    (sa, ta, ca), (sb, tb, cb) = res
+    assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1]) and ta == tb
+    assert all(v == 0 for v in ca.values()), ca
+    faces = 2 * p.grid.N * p.grid.M * 9
+    assert cb["exact_faces"] >= faces
+    assert cb["exact_faces"] == (cb["rarefaction_faces"] + cb["newton_faces"] + cb["bisect_faces"] +
+                                 cb["vacuum_faces"])
+    assert cb["capped"] == 0
+
+
 def test_sim_checkpoint_resume(dev, tmp_path):
     from asteriy_2d_amd.sim import Simulation
     p = problems.wind(256, 128)
