@@ -39,6 +39,12 @@ struct ExactCtx {
   int n;  // 1/gm = 2 gamma/(gamma - 1) when that is an integer in [1, 16] (5 for
           // gamma = 5/3, 7 for 7/5), else 0
   const unsigned char* mlp;  // learned-guess block (ast_mlp_t) in device memory, or null
+  // Gated launch (the runtime's learned mode): the kernel runs only while
+  // *gate == gate_mode, and ORs 1 into *strong when it meets a strong
+  // two-shock face (null: ungated, nothing flagged).
+  const uint32_t* gate;
+  uint32_t gate_mode;
+  uint32_t* strong;
 };
 
 static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
@@ -60,6 +66,9 @@ static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
   // (an FP32 gamma of 5/3 or 7/5 still counts: 1e-6 is its resolution)
   e.n = (fabs(ig - ni) <= 1e-6 * ig && ni >= 1.0 && ni <= 16.0) ? (int)ni : 0;
   e.mlp = (const unsigned char*)mlp;
+  e.gate = nullptr;
+  e.gate_mode = 0;
+  e.strong = nullptr;
   return e;
 }
 
@@ -100,6 +109,7 @@ constexpr int kNoFace = -3;
 struct FaceInfo {
   int pat, it;
   bool capped, island, learned;
+  bool strong;  // a strong two-shock face (the learned start's gate), when detected
 };
 
 AST_DEV uint32_t wcount(bool c) {
@@ -112,6 +122,14 @@ AST_DEV uint32_t wcount(bool c) {
 struct NoStats {
   AST_DEV void clear() {}
   AST_DEV void add(const struct FaceInfo&) {}
+  AST_DEV bool any_strong() const { return false; }
+};
+// Only whether the wave met a strong two-shock face (gated launches).
+struct StrongStats {
+  uint32_t strong;
+  AST_DEV void clear() { strong = 0; }
+  AST_DEV void add(const FaceInfo& f) { strong += wcount(f.strong); }
+  AST_DEV bool any_strong() const { return strong != 0; }
 };
 
 // Solver counters of the FP32 exact solver.  Face counts are per wave
@@ -120,13 +138,15 @@ struct NoStats {
 // the whole strip and cost the kernel a wave of occupancy.  add() runs in
 // converged code (every lane of the wave active), after the face calls.
 struct StatsF32 {
-  uint32_t exact, newton, bisect, raref, vacuum, capped, fp64, mlp;  // per wave
-  uint32_t newton_it, bisect_it;                                     // per lane
+  uint32_t exact, newton, bisect, raref, vacuum, capped, fp64, mlp, strong;  // per wave
+  uint32_t newton_it, bisect_it;                                             // per lane
   AST_DEV void clear() {
-    exact = newton = bisect = raref = vacuum = capped = fp64 = mlp = 0;
+    exact = newton = bisect = raref = vacuum = capped = fp64 = mlp = strong = 0;
     newton_it = bisect_it = 0;
   }
+  AST_DEV bool any_strong() const { return strong != 0; }
   AST_DEV void add(const FaceInfo& f) {
+    strong += wcount(f.strong);
     exact += wcount(f.pat != kNoFace);
     raref += wcount(f.pat == 0);
     vacuum += wcount(f.pat == -1);
@@ -271,7 +291,8 @@ AST_DEV double sound_speed_d(double g, float p, float r) { return sqrt(g * (doub
 // holds a strong two-shock face, the wave evaluates the learned guess for
 // all its lanes on MFMA (a wave-uniform branch) and those lanes start
 // newton from it.
-template <bool LEARNED>
+// DETECT: report strong two-shock faces (fi.strong) without the network.
+template <bool LEARNED, bool DETECT = false>
 AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
                         FaceInfo& fi, float guess = -1.f) {
   const float g = e.g, g1 = e.g1, g2 = e.g2, gm = e.gm;
@@ -288,15 +309,18 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   const float Urz = -e.c2g1 * cII * (1.f - q);
   const float Uvk = -e.c2g1 * (cII + cI);
   const float Udf = enI - enII;
-  if (LEARNED && e.mlp != nullptr) {
+  bool strong = false;
+  if constexpr (LEARNED || DETECT) {
     const float zI = rI * cI, zII = rII * cII;
     const float pvrs = (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII);
-    const bool want = (Udf > Uud) && (pvrs > kStrongShock * pII);
-    if (__builtin_amdgcn_ballot_w64(want) != 0) {  // wave-uniform
+    strong = (Udf > Uud) && (pvrs > kStrongShock * pII);
+  }
+  if (LEARNED && e.mlp != nullptr) {
+    if (__builtin_amdgcn_ballot_w64(strong) != 0) {  // wave-uniform
       float fe[mlp::IN];
       const float lp = mlp::features<float>(rI, pI, enI, rII, pII, enII, g, fe);
       const float y = mlp::forward_wave_bf16(e.mlp, fe);
-      if (want) guess = __expf(y + lp);
+      if (strong) guess = __expf(y + lp);
     }
   }
   int pat;
@@ -340,6 +364,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   fi.capped = capped;
   fi.island = island;
   fi.learned = learned;
+  fi.strong = strong;
 
   // sample the solution at w = 0, kernel.cu:234-544
   float en, pp, r, te;

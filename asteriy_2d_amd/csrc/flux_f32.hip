@@ -13,8 +13,9 @@ namespace f32 {
 
 constexpr int FNT = 256;
 
-// STATS: the variant that counts the exact solver's work into *stats
-template <int METHOD, bool STATS>
+// STATS: the variant that counts the exact solver's work into *stats;
+// LEARNED: the exact solver with the learned newton start compiled in
+template <int METHOD, bool STATS, bool LEARNED>
 __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __restrict__ in,
                                                    float n1, float n2, float krad,
                                                    ExactCtx ex, float* __restrict__ out,
@@ -44,8 +45,8 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
     else if constexpr (METHOD == AST_HLLC) f = hllc_f32(L, R, ex.g, ex.ig1, UU);
     else {
       FaceInfo fi;
-      f = exact_f32<true>(L, R, ex, UU, fi);
-      if (!valid) fi = FaceInfo{kNoFace, 0, false, false, false};
+      f = exact_f32<LEARNED>(L, R, ex, UU, fi);
+      if (!valid) fi = FaceInfo{kNoFace, 0, false, false, false, false};
       st.add(fi);
       if (info && valid) {
         info[2 * i] = fi.it;
@@ -68,12 +69,23 @@ static int launch(size_t n, const float* in, float n1, float n2, float krad, con
                   float* out, ast_stats_t* stats, int* info, hipStream_t stream) {
   size_t blocks = (n + FNT - 1) / FNT;
   if (blocks > 256 * 64) blocks = 256 * 64;
-  if (METHOD == AST_EXACT && stats != nullptr)
-    hipLaunchKernelGGL((flux_kernel<METHOD, true>), dim3((unsigned)blocks), dim3(FNT), 0, stream, n,
-                       in, n1, n2, krad, ex, out, stats, info);
-  else
-    hipLaunchKernelGGL((flux_kernel<METHOD, false>), dim3((unsigned)blocks), dim3(FNT), 0, stream, n,
-                       in, n1, n2, krad, ex, out, stats, info);
+  const bool st = METHOD == AST_EXACT && stats != nullptr;
+  if (METHOD == AST_EXACT && ex.mlp != nullptr) {
+    if constexpr (METHOD == AST_EXACT) {
+      if (st)
+        hipLaunchKernelGGL((flux_kernel<METHOD, true, true>), dim3((unsigned)blocks), dim3(FNT), 0,
+                           stream, n, in, n1, n2, krad, ex, out, stats, info);
+      else
+        hipLaunchKernelGGL((flux_kernel<METHOD, false, true>), dim3((unsigned)blocks), dim3(FNT), 0,
+                           stream, n, in, n1, n2, krad, ex, out, stats, info);
+    }
+  } else if (st) {
+    hipLaunchKernelGGL((flux_kernel<METHOD, true, false>), dim3((unsigned)blocks), dim3(FNT), 0,
+                       stream, n, in, n1, n2, krad, ex, out, stats, info);
+  } else {
+    hipLaunchKernelGGL((flux_kernel<METHOD, false, false>), dim3((unsigned)blocks), dim3(FNT), 0,
+                       stream, n, in, n1, n2, krad, ex, out, stats, info);
+  }
   AST_LAUNCH_CHECK("ast_flux_f32 launch");
   return AST_OK;
 }

@@ -15,8 +15,9 @@ __global__ void funk_time_kernel(double* T, double* T_do, double* TT, int* i) {
 }
 
 // funk_time on the runtime's time block with a spread dt minimum: T is the
-// minimum of the slots (time_slots.h), which are re-armed to 1e7 with T.
-// One wave; lane k folds slot k.
+// minimum of the slots (time_slots.h), which are re-armed to 1e7 with T; it
+// also selects the next step's learned mode (time_slots.h).  One wave; lane
+// k folds slot k.
 __global__ void funk_time_slots_kernel(double* T, int nslots) {
   const int k = threadIdx.x;
   double t = INFINITY;
@@ -33,6 +34,10 @@ __global__ void funk_time_slots_kernel(double* T, int nslots) {
     T[0] = 10000000;
     int* i = reinterpret_cast<int*>(T + 3);
     *i = *i + 1;
+    // learned mode of the next step: on while strong two-shock faces occur
+    uint32_t* gate = reinterpret_cast<uint32_t*>(T + kGateWord);
+    gate[0] = gate[1] ? 1u : 0u;
+    gate[1] = 0u;
   }
 }
 

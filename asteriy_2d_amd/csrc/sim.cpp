@@ -23,7 +23,7 @@ int ast_fail(int code, const char* fmt, ...);
 int ast_hip_fail(hipError_t e, const char* where);
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, int nslots, void* stream);
+                      ast_stats_t* stats, int krows, int nslots, uint32_t* gate, void* stream);
 int ast_funk_time_slots(double* T, int nslots, void* stream);
 extern "C" int ast_get_tile_rows(void);
 
@@ -68,7 +68,8 @@ static int launch_step(ast_sim_t* S, int from) {
   else  // dt spread over the slots of the time block, folded by the time advance
     rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
                            T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
-                           ast::kMaxSlots, S->stream);
+                           ast::kMaxSlots, reinterpret_cast<uint32_t*>(T + ast::kGateWord),
+                           S->stream);
   if (rc != AST_OK) return rc;
   if (S->precision == 1) return ast_funk_time_slots(T, ast::kMaxSlots, S->stream);
   return ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
@@ -289,7 +290,9 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
       int rc = S->precision == 1
                    ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                        (float*)u2, S->scal, S->scal + 1, S->method, S->mlp,
-                                       nullptr, kr, 0, S->stream)
+                                       nullptr, kr, 0,
+                                       reinterpret_cast<uint32_t*>(S->scal + ast::kGateWord),
+                                       S->stream)
                    : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
                                   (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
                                   nullptr, S->stream);
@@ -421,7 +424,8 @@ extern "C" int ast_sim_read_b31(ast_sim_t* S, const char* path) {
 // A run resumed from it continues bit for bit.
 struct CkptHeader {
   char magic[8];
-  int32_t version, N, M, precision, i, pad;
+  int32_t version, N, M, precision, i;
+  uint32_t gate_mode;  // learned mode of the next step (time_slots.h)
   double T, T_do, TT;
 };
 
@@ -436,6 +440,7 @@ extern "C" int ast_sim_save(ast_sim_t* S, const char* path) {
   h.precision = S->precision;
   int rc = ast_sim_get_time(S, &h.T, &h.T_do, &h.TT, &h.i);
   if (rc != AST_OK) return rc;
+  CK(hipMemcpy(&h.gate_mode, S->scal + ast::kGateWord, sizeof h.gate_mode, hipMemcpyDeviceToHost));
   const size_t bytes = S->cells * 2 * elem(S);
   std::vector<char> a(bytes), b(bytes);
   CK(hipMemcpy(a.data(), S->plane[2 * S->cur], bytes, hipMemcpyDeviceToHost));
@@ -468,5 +473,7 @@ extern "C" int ast_sim_load(ast_sim_t* S, const char* path) {
   S->cur = 0;
   CK(hipMemcpy(S->plane[0], a.data(), bytes, hipMemcpyHostToDevice));
   CK(hipMemcpy(S->plane[1], b.data(), bytes, hipMemcpyHostToDevice));
+  const uint32_t gate[2] = {h.gate_mode ? 1u : 0u, 0u};
+  CK(hipMemcpy(S->scal + ast::kGateWord, gate, sizeof gate, hipMemcpyHostToDevice));
   return ast_sim_set_time(S, h.T, h.T_do, h.TT, h.i);
 }

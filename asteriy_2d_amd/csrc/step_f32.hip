@@ -45,11 +45,16 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   int nwg, int nslots) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
+  // gated launch: this kernel's mode is not the selected one (grid-uniform)
+  if (ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
   const int wg = xcd_tile(blockIdx.x, nwg);
-  typename std::conditional<STATS, StatsF32, NoStats>::type st;
+  constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
+  typename std::conditional<STATS, StatsF32,
+                            typename std::conditional<kDetects, StrongStats, NoStats>::type>::type st;
   st.clear();
   const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, *T_do,
                                                           wg % nbands, wg / nbands, krows, st);
+  if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, T, nslots);
 }
@@ -98,7 +103,7 @@ static int dispatch_geom(const ast_grid_t& g, const Consts& c, const ExactCtx& e
                          const double* T_do, ast_stats_t* stats, int krows, int nslots,
                          hipStream_t st) {
   const bool axi = g.geometry == AST_AXISYMMETRIC;
-  if constexpr (METHOD == kExactLearned) {  // dispatched without a window only
+  if constexpr (METHOD == kExactLearned || METHOD == kExactDetect) {  // without a window only
     (void)window;
     return axi ? launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
                : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
@@ -111,9 +116,14 @@ static int dispatch_geom(const ast_grid_t& g, const Consts& c, const ExactCtx& e
                 : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
 }
 
+// gate (may be null): two words {mode, strong} of the runtime's learned mode.
+// With the exact method, a learned guess and no window, the step is then two
+// gated launches: the plain solver (mode 0), which flags strong two-shock
+// faces in `strong`, and the solver with the learned start (mode 1); the
+// time advance selects the next step's mode from the flag.
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, int nslots, void* stream) {
+                      ast_stats_t* stats, int krows, int nslots, uint32_t* gate, void* stream) {
   int rc = ast_grid_check(g);
   if (rc != AST_OK) return rc;
   if (!s || !u || !s2 || !u2 || !T || !T_do)
@@ -139,6 +149,16 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
     case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
     case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
     case AST_EXACT:  // the learned start only where it can run (no window: full waves)
+      if (mlp != nullptr && !window && gate != nullptr) {
+        ExactCtx e0 = ex, e1 = ex;
+        e0.gate = e1.gate = gate;
+        e0.strong = e1.strong = gate + 1;
+        e0.gate_mode = 0;
+        e1.gate_mode = 1;
+        rc = dispatch_geom<kExactDetect>(*g, c, e0, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+        if (rc != AST_OK) return rc;
+        return dispatch_geom<kExactLearned>(*g, c, e1, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+      }
       if (mlp != nullptr && !window)
         return dispatch_geom<kExactLearned>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
       return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
@@ -150,5 +170,5 @@ extern "C" int ast_step_f32(const ast_grid_t* g, const float* s, const float* u,
                             float* u2, double* T, const double* T_do, double* TT, int method,
                             const ast_mlp_t* mlp, ast_stats_t* stats, void* stream) {
   (void)TT;
-  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, 0, stream);
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, 0, nullptr, stream);
 }

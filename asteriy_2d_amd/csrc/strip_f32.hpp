@@ -155,16 +155,18 @@ static inline int check_positive_bc(const ast_grid_t& g) {
   return AST_OK;
 }
 
-// Kernel-internal method id: the exact solver with the learned newton start
-// compiled in (AST_EXACT kernels leave it out).
+// Kernel-internal method ids: the exact solver with the learned newton start
+// compiled in (AST_EXACT kernels leave it out), and the plain exact solver
+// that reports strong two-shock faces (the other half of a gated launch).
 constexpr int kExactLearned = AST_EXACT + 1;
+constexpr int kExactDetect = AST_EXACT + 2;
 
 template <int METHOD>
 AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
                         const ExactCtx& ex, FaceInfo& fi) {
   if constexpr (METHOD == AST_HLL) return hll_f32(L, R, c.g, c.ig1, UU);
   else if constexpr (METHOD == AST_HLLC) return hllc_f32(L, R, c.g, c.ig1, UU);
-  else return exact_f32<METHOD == kExactLearned>(L, R, ex, UU, fi);
+  else return exact_f32<METHOD == kExactLearned, METHOD == kExactDetect>(L, R, ex, UU, fi);
 }
 
 // One face in the x (NORMAL_X) or y direction: rotate to the face frame,
@@ -179,7 +181,7 @@ AST_DEV FluxF face(const Q& l, const Q& r, const Q& lc, const Q& rc, bool ho, co
   FaceInfo fi;
   fi.pat = kNoFace;
   fi.it = 0;
-  fi.capped = fi.island = fi.learned = false;
+  fi.capped = fi.island = fi.learned = fi.strong = false;
   FluxF f;
   if constexpr (WINDOW) {  // faces diverge here: no wave-level learned guess
     if (!ho) {

@@ -90,6 +90,70 @@ def _kernel_ms(grid, precision, method, stream, krows=None, reps=50):
     return e0.elapsed_time(e1) / reps
 
 
+def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
+    """HLL, HLLC and the exact solver over n seeded face pairs (the
+    microbenchmark distribution, 2% near vacuum), FP64 and FP32 (the exact one
+    also with the learned newton start): Gfaces/s over `reps` launches with the
+    inputs resident, HBM fraction at 104 / 52 B per face, and the exact
+    solvers' iterations per face from one counting launch."""
+    import ctypes
+
+    import torch
+
+    from asteriy_2d_amd import EXACT, HLL, HLLC, facegen, native, solver
+    dev = torch.device("cuda:0")
+    F64 = torch.from_numpy(facegen.faces(n)).to(dev)
+    F32 = F64.float()
+    o64 = torch.empty((5, n), dtype=torch.float64, device=dev)
+    o32 = torch.empty((5, n), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def call32(method, mlp, stats=None):
+        native.call("ast_flux_f32_info", int(method), n, F32.data_ptr(), 1.0, 0.0, 1.0, 5.0 / 3.0,
+                    0.05, ctypes.c_void_p(mlp.buffer.data_ptr()) if mlp is not None else None,
+                    o32.data_ptr(), stats.data_ptr() if stats is not None else None, None,
+                    ctypes.c_void_p(stream.cuda_stream))
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps  # ms per launch
+
+    res = {"faces": n}
+    for name, m in (("hll", HLL), ("hllc", HLLC), ("exact", EXACT)):
+        t64 = timed(lambda: solver.flux_batch(m, F64, out=o64, stream=stream))
+        t32 = timed(lambda: call32(m, None))
+        r = {"f64_gfaces": round(n / t64 / 1e6, 3), "f32_gfaces": round(n / t32 / 1e6, 3),
+             "f64_hbm_frac": round(104.0 * n / (t64 * 1e-3) / 8e12, 4),
+             "f32_hbm_frac": round(52.0 * n / (t32 * 1e-3) / 8e12, 4)}
+        if m == EXACT:
+            tl = timed(lambda: call32(m, mdl))
+            r["f32_learned_gfaces"] = round(n / tl / 1e6, 3)
+            its = {}
+            for tag, prec, mm in (("f64", 64, None), ("f32", 32, None), ("f32_learned", 32, mdl)):
+                st = solver.new_stats(dev)
+                if prec == 64:
+                    solver.flux_batch(m, F64, out=o64, stats=st, stream=stream)
+                else:
+                    call32(m, mm, st)
+                torch.cuda.synchronize()
+                d = native.stats_from_tensor(st)
+                its[tag] = {"newton_faces": d["newton_faces"],
+                            "newton_iters_per_face": round(d["newton_iters"] / max(1, d["newton_faces"]), 3),
+                            "bisect_iters_per_face": round(d["bisect_iters"] / max(1, d["bisect_faces"]), 3),
+                            "learned_faces": d["mlp_faces"], "capped": d["capped"]}
+            r["iterations"] = its
+        res[name] = r
+    del F64, F32, o64, o32
+    return res
+
+
 def _l1(a, b):
     import numpy as np
     return float(np.abs(a - b).sum() / np.abs(b).sum())
@@ -272,7 +336,11 @@ def main(argv=None):
         # -- exact solver at 2048x1024: FP32 (+FP64 islands) vs FP64 ----------
         Ke = 200
         pe = problems.wind(N_GRID, M_GRID, method=EXACT, steps=Ke)
+        from asteriy_2d_amd import learned
+        mdl = learned.LearnedGuess.load()
         me32, e32, _, sx, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, autotune=True, stats=True)
+        mel, el, _, sxl, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, mlp=mdl, autotune=True,
+                                      stats=True)
         me64, e64, _, s64x, _ = _run_sim(g, EXACT, "f64", pe.s, pe.u, Ke, stats=True)
         extra["exact_2048x1024"] = {
             "steps": Ke,
@@ -280,17 +348,19 @@ def main(argv=None):
             "fp64_gcells": round(cells * Ke / me64 / 1e6, 3),
             "speedup": round(me64 / me32, 3),
             "fp32_exact_vs_fp64_hllc": round(ms64 / K * Ke / me32, 3),
+            "fp32_learned_gcells": round(cells * Ke / mel / 1e6, 3),
+            "fp32_learned_speedup": round(me64 / mel, 3),
+            "fp32_learned_error_vs_fp64": _errors(el, e64),
+            "learned_faces": sxl["mlp_faces"],
             "error_vs_fp64": _errors(e32, e64),
             "newton_iters_per_face_fp32": round(sx["newton_iters"] / max(1, sx["newton_faces"]), 3),
             "newton_iters_per_face_fp64": round(s64x["newton_iters"] / max(1, s64x["newton_faces"]), 3),
             "fp64_island_faces": sx["fp64_faces"],
         }
         log(f"exact 2048: {extra['exact_2048x1024']}")
-        del e32, e64
+        del e32, e64, el
 
         # -- exact + learned newton start at 4096x2048 ------------------------
-        from asteriy_2d_amd import learned
-        mdl = learned.LearnedGuess.load()
         pc = problems.wind(4096, 2048, method=EXACT)
         gc = pc.grid
         Kc = 100
@@ -311,6 +381,10 @@ def main(argv=None):
         }
         log(f"exact+learned 4096: {extra['exact_learned_4096x2048']}")
         del c_m, c_64
+
+        # -- Riemann-flux microbenchmark: 2^24 face pairs --------------------
+        extra["flux_2e24"] = _flux_microbenchmark(mdl)
+        log(f"flux microbenchmark: {extra['flux_2e24']}")
 
         # -- batched sweep: 64 instances, one persistent launch per step --------
         from asteriy_2d_amd import batch as B

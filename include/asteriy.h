@@ -269,7 +269,12 @@ int ast_sim_get_time(ast_sim_t* sim, double* T, double* T_do, double* TT, int* i
  * variants of the exact-solver kernels run markedly slower). */
 int ast_sim_get_stats(ast_sim_t* sim, ast_stats_t* out, int reset);
 int ast_sim_collect_stats(ast_sim_t* sim, int on);
-/* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step. */
+/* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step.
+ * The network costs the solver kernel registers (occupancy) whether or not a
+ * face uses it, so the simulation runs it only where it can serve: each step
+ * launches the plain solver and the solver with the learned start, gated by
+ * a device word, and only one of them runs -- the learned one in the steps
+ * that follow a step with strong two-shock faces. */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
 /* Strip height (rows per wave) of the FP32 step; multiples of 4 take the
  * fast strip path. */

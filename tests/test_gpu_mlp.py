@@ -6,7 +6,8 @@ import os
 import numpy as np
 import pytest
 
-from asteriy_2d_amd import facegen
+from asteriy_2d_amd import facegen, learned, problems
+from asteriy_2d_amd import grid as G
 
 pytestmark = pytest.mark.gpu
 
@@ -139,3 +140,60 @@ def test_trained_guess_gain(dev):
     assert int((info1[:, 1] & 8).sum()) == 0  # never capped
     rel = ((out1 - out0).abs() / (out0.abs().max(0).values + 1e-300)).max().item()
     assert rel < 1e-8, rel
+
+
+def _colliding(N=128, M=64):
+    """Two streams colliding at z = 0.5 (Mach ~3.9 each): strong two-shock faces."""
+    g = G.Grid.from_extent(N, M, 0.0, 1.0, 0.5, krit=0.3, geometry=G.PLANAR,
+                           bc=(G.BC_OUTFLOW, G.BC_OUTFLOW, G.BC_REFLECT, G.BC_OUTFLOW))
+    x = g.x(np.arange(N))[None, :]
+    s = np.ones((M, N, 2))
+    u = np.zeros((M, N, 2))
+    u[..., 0] = np.where(x < 0.5, 5.0, -5.0)
+    return g, s, u
+
+
+def test_gated_learned_step_without_strong_faces_is_the_plain_step(dev):
+    """No strong two-shock face in the wind workload: the gated simulation
+    never switches to the learned kernel and steps bit for bit like the one
+    without a network."""
+    from asteriy_2d_amd.sim import Simulation
+    p = problems.wind(256, 128)
+    model = learned.LearnedGuess.load(device=dev)
+    out = []
+    for mlp in (None, model):
+        with Simulation(p.grid, G.EXACT, "f32", p.s, p.u) as sim:
+            if mlp is not None:
+                sim.set_mlp(mlp)
+            sim.collect_stats(True)
+            sim.run(12)
+            out.append((sim.state(), sim.time(), sim.stats()))
+    (sa, ta, ca), (sb, tb, cb) = out
+    assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1]) and ta == tb
+    assert cb["mlp_faces"] == 0
+
+
+def test_gated_learned_step_serves_strong_faces(dev):
+    """Colliding streams: after the first step flags strong two-shock faces
+    the learned kernel runs (its faces start newton from the network) and the
+    result agrees with the plain solver to the FP32 newton tolerance; graph
+    replay and eager launches agree bit for bit."""
+    from asteriy_2d_amd.sim import Simulation
+    g, s, u = _colliding()
+    model = learned.LearnedGuess.load(device=dev)
+    res = {}
+    for tag, mlp, graph in (("plain", None, True), ("learned", model, True),
+                            ("learned_eager", model, False)):
+        with Simulation(g, G.EXACT, "f32", s, u) as sim:
+            if mlp is not None:
+                sim.set_mlp(mlp)
+            sim.collect_stats(True)
+            sim.run(10, graph=graph)
+            res[tag] = (sim.state(), sim.stats())
+    (a, ca), (b, cb), (c, _) = res["plain"], res["learned"], res["learned_eager"]
+    assert ca["mlp_faces"] == 0 and cb["mlp_faces"] > 0, (ca, cb)
+    assert cb["newton_faces"] > 0
+    for k in (0, 1):
+        rel = np.abs(b[0][..., k] - a[0][..., k]).sum() / np.abs(a[0][..., k]).sum()
+        assert rel < 1e-5, (k, rel)
+    assert np.array_equal(b[0], c[0]) and np.array_equal(b[1], c[1])
