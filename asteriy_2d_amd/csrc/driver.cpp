@@ -5,12 +5,14 @@
 // through the graph-replaying runtime (ast_sim_*), prints progress in the
 // reference's form ("i = .., TT = .., dt = ..", kernel.cu:1531-1534, and the
 // wall time of a block of steps, kernel.cu:3965-3968) and writes the final
-// state in the reference's binary format (B31.bin, kernel.cu:4111-4148).
+// state in the reference's binary format (B31.bin, kernel.cu:4111-4148) and,
+// on request, its text outputs (Tecplot file of every 2nd node, the axis row,
+// kernel.cu:4127-4215).
 //
 //   asteriy_run --problem wind|sod|live [--N n --M m] [--method hll|hllc|exact]
 //               [--precision f64|f32] [--steps k] [--print-every k]
-//               [--out B31.bin] [--checkpoint file] [--resume file]
-//               [--mlp params.f32] [--autotune]
+//               [--out B31.bin] [--tecplot file] [--axis file]
+//               [--checkpoint file] [--resume file] [--mlp params.f32] [--autotune]
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -26,7 +28,8 @@
 namespace {
 
 struct Opts {
-  std::string problem = "wind", method = "hllc", precision = "f64", out, ckpt, resume, mlp;
+  std::string problem = "wind", method = "hllc", precision = "f64", out, ckpt, resume, mlp, tec,
+      axis;
   int N = 0, M = 0, steps = 200, every = 5000;
   bool autotune = false;
 };
@@ -40,7 +43,8 @@ void usage() {
   printf(
       "usage: asteriy_run --problem wind|sod|live [--N n --M m] [--method hll|hllc|exact]\n"
       "                   [--precision f64|f32] [--steps k] [--print-every k] [--out B31.bin]\n"
-      "                   [--checkpoint file] [--resume file] [--mlp params.f32] [--autotune]\n"
+      "                   [--tecplot file] [--axis file] [--checkpoint file] [--resume file]\n"
+      "                   [--mlp params.f32] [--autotune]\n"
       "  wind : stellar wind / counter-flow, z in [-400,400], r in [0,400], gamma 5/3, CFL 0.3\n"
       "  sod  : axisymmetric Sod tube, z in [0,1], r in [0,0.5], gamma 1.4, CFL 0.4\n"
       "  live : the configuration the reference's main() runs (Header.h), x in [0,80],\n"
@@ -167,6 +171,8 @@ int main(int argc, char** argv) {
     else if (a == "--steps") o.steps = atoi(next());
     else if (a == "--print-every") o.every = atoi(next());
     else if (a == "--out") o.out = next();
+    else if (a == "--tecplot") o.tec = next();
+    else if (a == "--axis") o.axis = next();
     else if (a == "--checkpoint") o.ckpt = next();
     else if (a == "--resume") o.resume = next();
     else if (a == "--mlp") o.mlp = next();
@@ -224,6 +230,8 @@ int main(int argc, char** argv) {
   printf("%d step - Time:  %.2f sec  (%.3f Gcell-updates/s)\n", o.steps, sec,
          o.steps * cells / sec / 1e9);
   if (!o.out.empty() && ast_sim_write_b31(sim, o.out.c_str()) != AST_OK) return die("write B31");
+  if (!o.tec.empty() && ast_sim_write_tecplot(sim, o.tec.c_str(), 2) != AST_OK) return die("write Tecplot");
+  if (!o.axis.empty() && ast_sim_write_axis(sim, o.axis.c_str()) != AST_OK) return die("write axis");
   if (!o.ckpt.empty() && ast_sim_save(sim, o.ckpt.c_str()) != AST_OK) return die("checkpoint");
   ast_sim_destroy(sim);
   if (mlp) (void)hipFree(mlp);

@@ -13,6 +13,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <cmath>
+#include <fstream>
 #include <new>
 #include <vector>
 
@@ -396,6 +398,70 @@ extern "C" int ast_sim_write_b31(ast_sim_t* S, const char* path) {
     }
   }
   fclose(f);
+  return AST_OK;
+}
+
+// Tecplot ASCII file of the state in the form of the reference's final
+// output (kernel.cu:4127-4129, 4170-4215): every stride-th node in x and y
+// (the reference's Nmin = 2), FEPOINT with quadrilateral elements (1-based
+// corner lists), variables X, Y, Ro, P, Vx, Vy, Mach, T = p/rho and Zav (the
+// vorticity dvy/dx - dvx/dy by central differences, 0 on the boundary rows
+// and columns), written with the default iostream formatting as there.
+extern "C" int ast_sim_write_tecplot(ast_sim_t* S, const char* path, int stride) {
+  if (!S || !path || stride < 1) return ast_fail(AST_ERR_ARG, "ast_sim_write_tecplot: bad argument");
+  const int N = S->g.N, M = S->g.M;
+  std::vector<double> s(S->cells * 2), u(S->cells * 2);
+  int rc = ast_sim_get_state(S, s.data(), u.data());
+  if (rc != AST_OK) return rc;
+  std::ofstream out(path);
+  if (!out) return ast_fail(AST_ERR_ARG, "ast_sim_write_tecplot: cannot open %s", path);
+  const int nn = (N + stride - 1) / stride, mm = (M + stride - 1) / stride;
+  out << "TITLE = \"HP\"  VARIABLES = \"X\", \"Y\", \"Ro\", \"P\", \"Vx\", \"Vy\", \"Mach\", "
+         "\"T\", \"Zav\", ZONE T = \"HP\", N = "
+      << nn * mm << " , E = " << (nn - 1) * (mm - 1) << ", F = FEPOINT, ET = quadrilateral\n";
+  const double g = S->g.gamma, dx = S->g.dx, dy = S->g.dy;
+  for (int m = 0; m < M; m += stride)
+    for (int n = 0; n < N; n += stride) {
+      const size_t k = (size_t)m * N + n;
+      const double rho = s[2 * k], p = s[2 * k + 1], vx = u[2 * k], vy = u[2 * k + 1];
+      double zav = 0.0;
+      if (n > 0 && m > 0 && n < N - 1 && m < M - 1)
+        zav = (u[2 * (k + 1) + 1] - u[2 * (k - 1) + 1]) / (2 * dx) -
+              (u[2 * (k + N)] - u[2 * (k - N)]) / (2 * dy);
+      double mach = 0.0, temp = 0.0;
+      if (rho > 0.0) {
+        mach = std::sqrt((vx * vx + vy * vy) / (g * p / rho));
+        temp = p / rho;
+      }
+      out << S->g.x_min + n * dx << " " << S->g.y_min + m * dy << " " << rho << " " << p << " " << vx
+          << " " << vy << " " << mach << " " << temp << " " << zav << "\n";
+    }
+  for (int m = 0; m + 1 < mm; m++)
+    for (int n = 0; n + 1 < nn; n++)
+      out << m * nn + n + 1 << " " << m * nn + n + 2 << " " << (m + 1) * nn + n + 2 << " "
+          << (m + 1) * nn + n + 1 << "\n";
+  out.close();
+  if (!out) return ast_fail(AST_ERR_ARG, "ast_sim_write_tecplot: write failed");
+  return AST_OK;
+}
+
+// The row next to the axis (m = 0) as text, the reference's "_param_y=0"
+// file (kernel.cu:4151-4164): x y rho p vx vy 1/c per line.
+extern "C" int ast_sim_write_axis(ast_sim_t* S, const char* path) {
+  if (!S || !path) return ast_fail(AST_ERR_ARG, "ast_sim_write_axis: null argument");
+  std::vector<double> s(S->cells * 2), u(S->cells * 2);
+  int rc = ast_sim_get_state(S, s.data(), u.data());
+  if (rc != AST_OK) return rc;
+  std::ofstream out(path);
+  if (!out) return ast_fail(AST_ERR_ARG, "ast_sim_write_axis: cannot open %s", path);
+  for (int n = 0; n < S->g.N; n++) {
+    const double rho = s[2 * n], p = s[2 * n + 1];
+    const double ss = rho > 0 ? 1.0 / std::sqrt(S->g.gamma * p / rho) : 0.0;
+    out << S->g.x_min + n * S->g.dx << " " << S->g.y_min << " " << rho << " " << p << " "
+        << u[2 * n] << " " << u[2 * n + 1] << " " << ss << "\n";
+  }
+  out.close();
+  if (!out) return ast_fail(AST_ERR_ARG, "ast_sim_write_axis: write failed");
   return AST_OK;
 }
 

@@ -36,6 +36,8 @@ SIM_SIGNATURES = {
     "ast_sim_sync": (_c_int, [_vp]),
     "ast_sim_write_b31": (_c_int, [_vp, ctypes.c_char_p]),
     "ast_sim_read_b31": (_c_int, [_vp, ctypes.c_char_p]),
+    "ast_sim_write_tecplot": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
+    "ast_sim_write_axis": (_c_int, [_vp, ctypes.c_char_p]),
     "ast_sim_save": (_c_int, [_vp, ctypes.c_char_p]),
     "ast_sim_load": (_c_int, [_vp, ctypes.c_char_p]),
 }
@@ -158,6 +160,12 @@ class Simulation:
 
     def read_b31(self, path: str):
         native.call("ast_sim_read_b31", self.h, path.encode())
+
+    def write_tecplot(self, path: str, stride: int = 2):
+        native.call("ast_sim_write_tecplot", self.h, path.encode(), int(stride))
+
+    def write_axis(self, path: str):
+        native.call("ast_sim_write_axis", self.h, path.encode())
 
     def save(self, path: str):
         native.call("ast_sim_save", self.h, path.encode())

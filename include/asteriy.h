@@ -294,6 +294,12 @@ int ast_sim_sync(ast_sim_t* sim);
  * 4111-4148 write): per cell in k order, 6 doubles x, y, rho, p, vx, vy. */
 int ast_sim_write_b31(ast_sim_t* sim, const char* path);
 int ast_sim_read_b31(ast_sim_t* sim, const char* path);
+/* The reference's text outputs of a run (kernel.cu:4127-4215): a Tecplot
+ * FEPOINT quadrilateral file of every stride-th node (the reference uses 2)
+ * with X, Y, Ro, P, Vx, Vy, Mach, T = p/rho, Zav (vorticity), and the row
+ * next to the axis (x y rho p vx vy 1/c per line). */
+int ast_sim_write_tecplot(ast_sim_t* sim, const char* path, int stride);
+int ast_sim_write_axis(ast_sim_t* sim, const char* path);
 /* Checkpoint with the time state (T, T_do, TT, i) the reference does not
  * save: a resumed run continues bit for bit. */
 int ast_sim_save(ast_sim_t* sim, const char* path);

@@ -248,6 +248,52 @@ def test_sim_b31_roundtrip(dev, tmp_path):
     assert np.array_equal(s2, s) and np.array_equal(u2, u)
 
 
+def test_sim_text_outputs(dev, tmp_path):
+    """Tecplot file (every 2nd node, quadrilaterals, Mach/T/vorticity) and the
+    axis row, the reference's text outputs (kernel.cu:4127-4215), checked
+    against the state read back and the same quantities formed with numpy."""
+    from asteriy_2d_amd.sim import Simulation
+    N, M = 65, 34  # odd N: the last column is a sampled node, the last row is not
+    p = problems.wind(N, M)
+    with Simulation(p.grid, G.HLLC, "f64", p.s, p.u) as a:
+        a.run(5)
+        a.write_tecplot(str(tmp_path / "t.dat"), 2)
+        a.write_axis(str(tmp_path / "axis.txt"))
+        s, u = a.state()
+    lines = (tmp_path / "t.dat").read_text().splitlines()
+    nn, mm = (N + 1) // 2, (M + 1) // 2
+    assert lines[0].startswith('TITLE = "HP"  VARIABLES = "X", "Y", "Ro", "P", "Vx", "Vy", '
+                               '"Mach", "T", "Zav", ZONE T = "HP"')
+    assert f"N = {nn * mm} , E = {(nn - 1) * (mm - 1)}, F = FEPOINT" in lines[0]
+    nodes = np.array([[float(v) for v in l.split()] for l in lines[1:1 + nn * mm]])
+    quads = np.array([[int(v) for v in l.split()] for l in lines[1 + nn * mm:]])
+    assert nodes.shape == (nn * mm, 9) and quads.shape == ((nn - 1) * (mm - 1), 4)
+    g = p.grid
+    mi, ni = np.meshgrid(np.arange(0, M, 2), np.arange(0, N, 2), indexing="ij")
+    rho, pr = s[mi, ni, 0], s[mi, ni, 1]
+    vx, vy = u[mi, ni, 0], u[mi, ni, 1]
+    zav = np.zeros_like(rho)
+    inner = (ni > 0) & (mi > 0) & (ni < N - 1) & (mi < M - 1)
+    zc = ((u[mi, np.minimum(ni + 1, N - 1), 1] - u[mi, np.maximum(ni - 1, 0), 1]) / (2 * g.dx)
+          - (u[np.minimum(mi + 1, M - 1), ni, 0] - u[np.maximum(mi - 1, 0), ni, 0]) / (2 * g.dy))
+    zav[inner] = zc[inner]
+    want = np.stack([g.x_min + ni * g.dx, g.y_min + mi * g.dy, rho, pr, vx, vy,
+                     np.sqrt((vx * vx + vy * vy) / (g.gamma * pr / rho)), pr / rho, zav],
+                    axis=-1).reshape(-1, 9)
+    # default iostream formatting: 6 significant digits
+    assert np.allclose(nodes, want, rtol=1e-5, atol=1e-5 * np.abs(want).max(axis=0))
+    # 1-based corners, counter-clockwise: (n,m) (n+1,m) (n+1,m+1) (n,m+1)
+    first = np.arange(mm - 1)[:, None] * nn + np.arange(nn - 1)[None, :] + 1
+    assert np.array_equal(quads, np.stack([first, first + 1, first + nn + 1, first + nn],
+                                          axis=-1).reshape(-1, 4))
+    axis = np.loadtxt(tmp_path / "axis.txt")
+    want = np.stack([g.x_min + np.arange(N) * g.dx, np.full(N, g.y_min), s[0, :, 0], s[0, :, 1],
+                     u[0, :, 0], u[0, :, 1], 1 / np.sqrt(g.gamma * s[0, :, 1] / s[0, :, 0])],
+                    axis=-1)
+    assert axis.shape == (N, 7)
+    assert np.allclose(axis, want, rtol=1e-5, atol=1e-5 * np.abs(want).max(axis=0))
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_driver_matches_runtime(dev, tmp_path, precision):
     """asteriy_run (the reference's main() over the C-ABI) writes the same
@@ -257,7 +303,8 @@ def test_driver_matches_runtime(dev, tmp_path, precision):
     from asteriy_2d_amd.sim import Simulation
     out = tmp_path / "B31.bin"
     r = subprocess.run([build.EXE, "--problem", "sod", "--steps", "60", "--print-every", "30",
-                        "--precision", precision, "--method", "hll", "--out", str(out)],
+                        "--precision", precision, "--method", "hll", "--out", str(out),
+                        "--axis", str(tmp_path / "axis.txt")],
                        capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stderr
@@ -269,6 +316,9 @@ def test_driver_matches_runtime(dev, tmp_path, precision):
         s, u = sim.state()
     assert np.array_equal(rec[:, 2], s[..., 0].ravel())
     assert np.array_equal(rec[:, 5], u[..., 1].ravel())
+    axis = np.loadtxt(tmp_path / "axis.txt")
+    assert axis.shape == (p.grid.N, 7)
+    assert np.allclose(axis[:, 2], s[0, :, 0], rtol=1e-5)
 
 
 def test_driver_reference_live_configuration(dev):
