@@ -133,9 +133,9 @@ AST_DEV void block_min_to(double tmin, double* red, double* T) {
 }
 
 // block_min_to for a tmin that is already uniform across each wave.  With
-// nslots > 0 workgroup b lowers slot (b mod nslots) of the time block
-// (time_slots.h) instead of *T, so that no single address takes every
-// workgroup's atomic; the time advance folds the slots (funk_time_slots).
+// nslots > 0, T is a set of dt slots (time_slots.h) and workgroup b lowers
+// slot (b mod nslots), so that no single address takes every workgroup's
+// atomic; the time advance folds the slots.
 template <int NT>
 AST_DEV void block_min_uniform(double tmin, double* red, double* T, int nslots) {
   constexpr int NW = NT / 64;
@@ -146,10 +146,48 @@ AST_DEV void block_min_uniform(double tmin, double* red, double* T, int nslots) 
     double t = red[0];
 #pragma unroll
     for (int i = 1; i < NW; i++) t = fmin(t, red[i]);
-    double* dst = nslots > 0 ? T + kSlot0 + kSlotStride * (int)(blockIdx.x % (unsigned)nslots) : T;
+    double* dst = nslots > 0 ? T + kSlotStride * (int)(blockIdx.x % (unsigned)nslots) : T;
     if (t < INFINITY)
       atomicMin(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)__double_as_longlong(t));
   }
+}
+
+// The time arguments of one step launch (time_slots.h): prev, the slot set
+// of the previous step (null: dt is *T_do); cur, the set this step lowers
+// (null: *T itself); rst, a set to clear (may be null).
+struct KTime {
+  const double* prev;
+  double* cur;
+  double* rst;
+  int nslots;
+};
+
+// dt of this step.  With a previous slot set, dt is the minimum of its slots
+// (every wave folds them itself: no workgroup waits for another) and wave 0
+// of workgroup 0 performs the previous step's time advance, funk_time of
+// kernel.cu:1525-1536 (T_do = that minimum, TT += T_do, i += 1; T stays at
+// its reset value), and clears the set `rst` for the next step.
+AST_DEV double step_dt(double* T, const double* T_do, const KTime& kt) {
+  const int lane = threadIdx.x & 63;
+  double t;
+  if (kt.prev != nullptr) {
+    t = wave_min(lane < kt.nslots ? kt.prev[kSlotStride * lane] : INFINITY);
+  } else {
+    t = *T_do;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    if (kt.rst != nullptr) {
+      if (lane < kt.nslots) kt.rst[kSlotStride * lane] = kSlotEmpty;
+      if (lane == 0) reinterpret_cast<uint32_t*>(kt.rst + 1)[0] = 0u;
+    }
+    if (kt.prev != nullptr && lane == 0) {
+      T[1] = t;
+      T[2] = T[2] + t;
+      int* i = reinterpret_cast<int*>(T + 3);
+      *i = *i + 1;
+    }
+  }
+  return t;
 }
 
 // Adds the block's solver counters to *stats (device).

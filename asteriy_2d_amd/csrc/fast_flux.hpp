@@ -26,9 +26,11 @@ struct FluxF {
 
 AST_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 AST_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-// min / max as one v_med3_f32 against -inf / +inf (no operand canonicalisation)
-AST_DEV float fmin3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -INFINITY); }
-AST_DEV float fmax3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, INFINITY); }
+// min / max of finite values as one v_med3_f32 against -FLT_MAX / FLT_MAX
+// (a plain min/max of a value the compiler cannot prove canonical costs an
+// extra canonicalising instruction)
+AST_DEV float fmin3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -3.40282347e38f); }
+AST_DEV float fmax3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 3.40282347e38f); }
 
 // Physical flux of one side in the face frame.
 AST_DEV FluxF phys_flux(const PrimF& q, float E) {
@@ -42,7 +44,7 @@ AST_DEV FluxF phys_flux(const PrimF& q, float E) {
 
 // HLL with the reference's Davis speeds, kernel.cu:1436-1443, 1505-1519.
 AST_DEV FluxF hll_f32(const PrimF& L, const PrimF& R, float g, float ig1, float& UU) {
-  const float c2 = fmax3(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
+  const float c2 = g * fmax3(L.p * frcp(L.r), R.p * frcp(R.r));
   const float cm = fsqrt(c2);
   const float DL = fmin3(L.u, R.u) - cm;
   const float DR = fmax3(L.u, R.u) + cm;
@@ -79,7 +81,7 @@ AST_DEV FluxF hll_f32(const PrimF& L, const PrimF& R, float g, float ig1, float&
 // reference's energy uses the average of the two sides' p*, equal to p* in
 // exact arithmetic (that equality is what defines S_M).
 AST_DEV FluxF hllc_f32(const PrimF& L, const PrimF& R, float g, float ig1, float& UU) {
-  const float c2 = fmax3(g * L.p * frcp(L.r), g * R.p * frcp(R.r));
+  const float c2 = g * fmax3(L.p * frcp(L.r), R.p * frcp(R.r));
   const float cm = fsqrt(c2);
   const float SL = fmin3(L.u, R.u) - cm;                 // Gas_Dinamic.cu:1636
   const float SR = fmax3(L.u, R.u) + cm;                 // Gas_Dinamic.cu:1637

@@ -14,30 +14,33 @@ __global__ void funk_time_kernel(double* T, double* T_do, double* TT, int* i) {
   }
 }
 
-// funk_time on the runtime's time block with a spread dt minimum: T is the
-// minimum of the slots (time_slots.h), which are re-armed to 1e7 with T; it
-// also selects the next step's learned mode (time_slots.h).  One wave; lane
-// k folds slot k.
-__global__ void funk_time_slots_kernel(double* T, int nslots) {
+// The time advance that closes a run of steps on the runtime's time block
+// (time_slots.h): funk_time with T = the minimum of the slots of set `last`
+// (the last step's), the learned mode of the next step from that set's
+// strong flag, and every set cleared for the next run.  One wave; lane k
+// folds and clears slot k.
+__global__ void funk_time_fold_kernel(double* T, int last, int nslots) {
   const int k = threadIdx.x;
-  double t = INFINITY;
-  if (k < nslots) {
-    double* slot = T + kSlot0 + kSlotStride * k;
-    t = *slot;
-    *slot = 10000000;
-  }
+  double* L = T + kSlot0 + kSetDoubles * last;
+  double t = k < nslots ? L[kSlotStride * k] : INFINITY;
+  const uint32_t strong = reinterpret_cast<const uint32_t*>(L + 1)[0];
   t = wave_min(t);
   if (k == 0) {
     t = fmin(t, *T);  // a step launched without slots lowers T itself
     T[1] = t;
     T[2] = T[2] + t;
-    T[0] = 10000000;
+    T[0] = kSlotEmpty;
     int* i = reinterpret_cast<int*>(T + 3);
     *i = *i + 1;
     // learned mode of the next step: on while strong two-shock faces occur
     uint32_t* gate = reinterpret_cast<uint32_t*>(T + kGateWord);
-    gate[0] = gate[1] ? 1u : 0u;
+    gate[0] = strong ? 1u : 0u;
     gate[1] = 0u;
+  }
+  for (int j = 0; j < kSets; j++) {
+    double* S = T + kSlot0 + kSetDoubles * j;
+    if (k < nslots) S[kSlotStride * k] = kSlotEmpty;
+    if (k == 0) reinterpret_cast<uint32_t*>(S + 1)[0] = 0u;
   }
 }
 
@@ -120,13 +123,13 @@ __global__ void sums_pass2(const double* work, double* out) {
 
 }  // namespace ast
 
-// The runtime's time advance over a time block with `nslots` dt slots.
-int ast_funk_time_slots(double* T, int nslots, void* stream) {
-  if (!T || nslots < 0 || nslots > ast::kMaxSlots)
-    return ast_fail(AST_ERR_ARG, "ast_funk_time_slots: bad argument");
-  hipLaunchKernelGGL(ast::funk_time_slots_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, T,
-                     nslots);
-  AST_LAUNCH_CHECK("ast_funk_time_slots launch");
+// The time advance closing a run whose last step lowered slot set `last`.
+int ast_funk_time_fold(double* T, int last, int nslots, void* stream) {
+  if (!T || last < 0 || nslots < 1 || nslots > ast::kMaxSlots)
+    return ast_fail(AST_ERR_ARG, "ast_funk_time_fold: bad argument");
+  hipLaunchKernelGGL(ast::funk_time_fold_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, T,
+                     last % ast::kSets, nslots);
+  AST_LAUNCH_CHECK("ast_funk_time_fold launch");
   return AST_OK;
 }
 

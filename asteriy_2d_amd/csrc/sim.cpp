@@ -3,11 +3,13 @@
 // plane arguments) as a native object.
 //
 // The reference launches, checks and device-synchronises after every kernel
-// (4 launches + 4 blocking syncs per 2 steps).  Here the two-step ping-pong
-// (step A->B, funk_time, step B->A, funk_time) is captured once into a HIP
-// graph on the simulation's own stream and replayed; dt never leaves the
-// device (T, T_do, TT, i live in one device block, as in kernel.cu:3290-3293).
-// State is held in FP64 (the reference layout) or FP32 planes.
+// (4 launches + 4 blocking syncs per 2 steps).  Here a run of kChunk steps
+// (ping-pong A->B->A...) is captured once into a HIP graph on the
+// simulation's own stream and replayed; dt never leaves the device (T, T_do,
+// TT, i live in one device block, as in kernel.cu:3290-3293).  State is held
+// in FP64 (the reference layout: each step is Kernel_TVD + funk_time) or FP32
+// planes (each step is one kernel that also performs the previous step's time
+// advance, time_slots.h; one small kernel closes the run).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -25,8 +27,8 @@ int ast_fail(int code, const char* fmt, ...);
 int ast_hip_fail(hipError_t e, const char* where);
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, int nslots, uint32_t* gate, void* stream);
-int ast_funk_time_slots(double* T, int nslots, void* stream);
+                      ast_stats_t* stats, int krows, const ast::StepTime* tm, void* stream);
+int ast_funk_time_fold(double* T, int last, int nslots, void* stream);
 extern "C" int ast_get_tile_rows(void);
 
 #define CK(call)                                              \
@@ -57,24 +59,56 @@ typedef struct ast_sim ast_sim_t;
 
 static size_t elem(const ast_sim_t* s) { return s->precision ? sizeof(float) : sizeof(double); }
 
-static int launch_step(ast_sim_t* S, int from) {
-  void* s = S->plane[2 * from];
-  void* u = S->plane[2 * from + 1];
-  void* s2 = S->plane[2 * (1 - from)];
-  void* u2 = S->plane[2 * (1 - from) + 1];
+// steps per captured graph (even: a replay ends on the plane it starts from)
+constexpr int kChunk = 20;
+
+static ast::StepTime step_time(ast_sim_t* S, int j) {
   double* T = S->scal;
-  int rc;
-  if (S->precision == 0)
-    rc = ast_step_f64(&S->g, (const double*)s, (const double*)u, (double*)s2, (double*)u2, T,
-                      T + 1, T + 2, S->method, S->collect ? S->stats : nullptr, S->stream);
-  else  // dt spread over the slots of the time block, folded by the time advance
-    rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
-                           T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
-                           ast::kMaxSlots, reinterpret_cast<uint32_t*>(T + ast::kGateWord),
-                           S->stream);
-  if (rc != AST_OK) return rc;
-  if (S->precision == 1) return ast_funk_time_slots(T, ast::kMaxSlots, S->stream);
-  return ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
+  ast::StepTime tm;
+  tm.nslots = ast::kMaxSlots;
+  tm.prev = j > 0 ? ast::slot_set(T, j - 1) : nullptr;
+  tm.cur = ast::slot_set(T, j);
+  tm.rst = ast::slot_set(T, j + 1);
+  tm.T = T;
+  tm.gate = reinterpret_cast<uint32_t*>(T + ast::kGateWord);
+  return tm;
+}
+
+// n steps from plane `from`.  FP32: step j lowers slot set j (mod 3), takes
+// its dt from set j-1 and advances the time; a fold of the last set closes
+// the run, so between runs the time block reads as after funk_time.
+static int launch_steps(ast_sim_t* S, int from, int n) {
+  double* T = S->scal;
+  for (int j = 0; j < n; j++) {
+    const int f = from ^ (j & 1);
+    void* s = S->plane[2 * f];
+    void* u = S->plane[2 * f + 1];
+    void* s2 = S->plane[2 * (1 - f)];
+    void* u2 = S->plane[2 * (1 - f) + 1];
+    int rc;
+    if (S->precision == 0) {
+      rc = ast_step_f64(&S->g, (const double*)s, (const double*)u, (double*)s2, (double*)u2, T,
+                        T + 1, T + 2, S->method, S->collect ? S->stats : nullptr, S->stream);
+      if (rc == AST_OK) rc = ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
+    } else {
+      const ast::StepTime tm = step_time(S, j);
+      rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
+                             T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
+                             &tm, S->stream);
+    }
+    if (rc != AST_OK) return rc;
+  }
+  if (S->precision == 1 && n > 0) return ast_funk_time_fold(T, n - 1, ast::kMaxSlots, S->stream);
+  return AST_OK;
+}
+
+// The slot sets as a run leaves them (every slot at funk_time's reset value).
+static int clear_slot_sets(ast_sim_t* S) {
+  std::vector<double> v(ast::kTimeBlockDoubles - ast::kSlot0, 0.0);
+  for (int j = 0; j < ast::kSets; j++)
+    for (int k = 0; k < ast::kMaxSlots; k++) v[j * ast::kSetDoubles + ast::kSlotStride * k] = ast::kSlotEmpty;
+  CK(hipMemcpy(S->scal + ast::kSlot0, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  return AST_OK;
 }
 
 static void drop_graph(ast_sim_t* S) {
@@ -117,11 +151,9 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
     return ast_hip_fail(e, "ast_sim_create: allocation");
   }
   // kernel.cu:3321-3324: T = 1e7, T_do = 1e-9, TT = 0, i = 0; dt slots 1e7
-  std::vector<double> init(ast::kTimeBlockDoubles, 0.0);
-  init[0] = 1e7;
-  init[1] = 1e-9;
-  for (int k = 0; k < ast::kMaxSlots; k++) init[ast::kSlot0 + ast::kSlotStride * k] = 1e7;
-  CK(hipMemcpy(S->scal, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice));
+  double init[ast::kSlot0] = {1e7, 1e-9};
+  CK(hipMemcpy(S->scal, init, sizeof init, hipMemcpyHostToDevice));
+  if ((rc = clear_slot_sets(S)) != AST_OK) return rc;
   CK(hipMemset(S->stats, 0, sizeof(ast_stats_t)));
   *out = S;
   return AST_OK;
@@ -283,6 +315,8 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   void* u2 = S->plane[2 * (1 - from) + 1];
   int best = S->krows;
   float best_t = 1e30f;
+  // the first step of a run: dt from T_do, the learned mode from its word
+  const ast::StepTime tm = step_time(S, 0);
   const int n = S->precision == 1 ? ncand : 1;
   for (int c = 0; c < n; c++) {
     const int kr = S->precision == 1 ? candidates[c] : 0;
@@ -292,9 +326,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
       int rc = S->precision == 1
                    ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                        (float*)u2, S->scal, S->scal + 1, S->method, S->mlp,
-                                       nullptr, kr, 0,
-                                       reinterpret_cast<uint32_t*>(S->scal + ast::kGateWord),
-                                       S->stream)
+                                       nullptr, kr, &tm, S->stream)
                    : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
                                   (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
                                   nullptr, S->stream);
@@ -313,6 +345,8 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   CK(hipMemcpy(S->scal, saved, sizeof saved, hipMemcpyHostToDevice));
+  int rc = clear_slot_sets(S);
+  if (rc != AST_OK) return rc;
   if (S->precision == 1 && best != S->krows) {
     S->krows = best;
     drop_graph(S);
@@ -325,8 +359,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
 static int build_graph(ast_sim_t* S) {
   drop_graph(S);
   CK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-  int rc = launch_step(S, 0);
-  if (rc == AST_OK) rc = launch_step(S, 1);
+  int rc = launch_steps(S, 0, kChunk);
   hipGraph_t gr = nullptr;
   hipError_t e = hipStreamEndCapture(S->stream, &gr);
   if (rc != AST_OK) {
@@ -341,30 +374,29 @@ static int build_graph(ast_sim_t* S) {
 }
 
 // Advances `steps` steps asynchronously on the simulation stream.  With
-// use_graph, pairs of steps replay the captured graph (the graph always
-// starts from plane 0, so an odd position first takes one eager step).
+// use_graph, runs of kChunk steps replay the captured graph (which starts
+// from plane 0, so an odd position first takes one eager step); the rest
+// are launched directly.  Both give the same result bit for bit.
 extern "C" int ast_sim_run(ast_sim_t* S, int steps, int use_graph) {
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_run: null sim");
   if (steps < 0) return ast_fail(AST_ERR_ARG, "ast_sim_run: steps %d", steps);
   int rc;
   if (!use_graph) {
-    for (int k = 0; k < steps; k++) {
-      if ((rc = launch_step(S, S->cur)) != AST_OK) return rc;
-      S->cur ^= 1;
-    }
+    if ((rc = launch_steps(S, S->cur, steps)) != AST_OK) return rc;
+    S->cur ^= steps & 1;
     return AST_OK;
   }
   if (S->cur == 1 && steps > 0) {
-    if ((rc = launch_step(S, 1)) != AST_OK) return rc;
+    if ((rc = launch_steps(S, 1, 1)) != AST_OK) return rc;
     S->cur = 0;
     steps--;
   }
-  if (steps >= 2 && !S->graph_ok)
+  if (steps >= kChunk && !S->graph_ok)
     if ((rc = build_graph(S)) != AST_OK) return rc;
-  for (int k = 0; k + 1 < steps; k += 2) CK(hipGraphLaunch(S->exec, S->stream));
-  if (steps & 1) {
-    if ((rc = launch_step(S, 0)) != AST_OK) return rc;
-    S->cur = 1;
+  for (; steps >= kChunk; steps -= kChunk) CK(hipGraphLaunch(S->exec, S->stream));
+  if (steps > 0) {
+    if ((rc = launch_steps(S, 0, steps)) != AST_OK) return rc;
+    S->cur = steps & 1;
   }
   return AST_OK;
 }

@@ -22,7 +22,8 @@
 //    the wave reduces them (shuffles), dt = krit*min(dx/max_x, dy/max_y) in
 //    FP64, one 64-bit atomicMin per workgroup (deterministic).  The runtime
 //    spreads these atomics over 64 slots (one address would serialise
-//    them) and folds the slots when it advances the time.
+//    them), and the next step folds the slots into its own dt and advances
+//    the time itself (time_slots.h): a step is one kernel.
 //  * the exact solver runs in FP32 with its p* solve promoted to FP64 near
 //    vacuum and in strong rarefactions, optionally seeded by the learned
 //    guess (exact_f32.hpp).
@@ -40,23 +41,24 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   const float2* __restrict__ u,
                                                   float2* __restrict__ s2,
                                                   float2* __restrict__ u2, double* T,
-                                                  const double* __restrict__ T_do,
+                                                  const double* __restrict__ T_do, KTime kt,
                                                   ast_stats_t* stats, int nbands, int krows,
-                                                  int nwg, int nslots) {
+                                                  int nwg) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
   // gated launch: this kernel's mode is not the selected one (grid-uniform)
   if (ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
+  const double tdo = step_dt(T, T_do, kt);
   const int wg = xcd_tile(blockIdx.x, nwg);
   constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
   typename std::conditional<STATS, StatsF32,
                             typename std::conditional<kDetects, StrongStats, NoStats>::type>::type st;
   st.clear();
-  const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, *T_do,
+  const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo,
                                                           wg % nbands, wg / nbands, krows, st);
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
-  block_min_uniform<NT>(tmin, red, T, nslots);
+  block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
 }
 
 }  // namespace f32
@@ -77,7 +79,7 @@ extern "C" int ast_get_tile_rows(void) { return g_krows; }
 template <int METHOD, int GEOM, bool WINDOW>
 static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
                       const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
-                      ast_stats_t* stats, int krows, int nslots, hipStream_t stream) {
+                      const KTime& kt, ast_stats_t* stats, int krows, hipStream_t stream) {
   const int nbands = (g.N + WOUT - 1) / WOUT;
   const int nstrips = (g.M + krows - 1) / krows;
   const int nsg = (nstrips + WPB - 1) / WPB;
@@ -86,12 +88,12 @@ static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, 
   if (METHOD >= AST_EXACT && stats != nullptr) {
     if constexpr (METHOD >= AST_EXACT)
       hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3((unsigned)nwg), dim3(NT), 0,
-                         stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg,
-                         nslots);
+                         stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
+                         (int)nwg);
   } else {
     hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false>), dim3((unsigned)nwg), dim3(NT), 0,
-                       stream, g, c, ex, s, u, s2, u2, T, T_do, stats, nbands, krows, (int)nwg,
-                       nslots);
+                       stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
+                       (int)nwg);
   }
   AST_LAUNCH_CHECK("ast_step_f32 launch");
   return AST_OK;
@@ -100,30 +102,34 @@ static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, 
 template <int METHOD>
 static int dispatch_geom(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, bool window,
                          const float2* s, const float2* u, float2* s2, float2* u2, double* T,
-                         const double* T_do, ast_stats_t* stats, int krows, int nslots,
+                         const double* T_do, const KTime& kt, ast_stats_t* stats, int krows,
                          hipStream_t st) {
   const bool axi = g.geometry == AST_AXISYMMETRIC;
   if constexpr (METHOD == kExactLearned || METHOD == kExactDetect) {  // without a window only
     (void)window;
-    return axi ? launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
-               : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
+    return axi ? launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, kt, stats, krows, st)
+               : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, kt, stats, krows, st);
   }
   if (axi) {
-    return window ? launch_f32<METHOD, AST_AXISYMMETRIC, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
-                  : launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
+    return window ? launch_f32<METHOD, AST_AXISYMMETRIC, true>(g, c, ex, s, u, s2, u2, T, T_do, kt, stats, krows, st)
+                  : launch_f32<METHOD, AST_AXISYMMETRIC, false>(g, c, ex, s, u, s2, u2, T, T_do, kt, stats, krows, st);
   }
-  return window ? launch_f32<METHOD, AST_PLANAR, true>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st)
-                : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, stats, krows, nslots, st);
+  return window ? launch_f32<METHOD, AST_PLANAR, true>(g, c, ex, s, u, s2, u2, T, T_do, kt, stats, krows, st)
+                : launch_f32<METHOD, AST_PLANAR, false>(g, c, ex, s, u, s2, u2, T, T_do, kt, stats, krows, st);
 }
 
-// gate (may be null): two words {mode, strong} of the runtime's learned mode.
-// With the exact method, a learned guess and no window, the step is then two
-// gated launches: the plain solver (mode 0), which flags strong two-shock
-// faces in `strong`, and the solver with the learned start (mode 1); the
-// time advance selects the next step's mode from the flag.
+// tm (may be null): the runtime's time protocol (time_slots.h).  Null: the
+// reference protocol (the step lowers *T, dt is *T_do, a learned guess is
+// always on).  Otherwise the step lowers the slots of tm->cur and, with
+// tm->prev, takes its dt from the previous step's slots and advances the
+// time itself.  With the exact method, a learned guess, no window and
+// tm->gate, the step is two gated launches: the plain solver (mode 0), which
+// flags strong two-shock faces in the strong word of tm->cur, and the solver
+// with the learned start (mode 1); the mode is the strong flag of tm->prev,
+// or tm->gate[0] without one.
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, int nslots, uint32_t* gate, void* stream) {
+                      ast_stats_t* stats, int krows, const StepTime* tm, void* stream) {
   int rc = ast_grid_check(g);
   if (rc != AST_OK) return rc;
   if (!s || !u || !s2 || !u2 || !T || !T_do)
@@ -131,7 +137,12 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   if (s == s2 || u == u2 || s == u2 || u == s2)
     return ast_fail(AST_ERR_ARG, "ast_step_f32: output planes must not alias the input planes");
   if (krows < 1) return ast_fail(AST_ERR_ARG, "ast_step_f32: krows %d", krows);
-  if (nslots < 0 || nslots > kMaxSlots) return ast_fail(AST_ERR_ARG, "ast_step_f32: %d dt slots", nslots);
+  KTime kt = {nullptr, nullptr, nullptr, 0};
+  if (tm != nullptr) {
+    if (tm->nslots < 1 || tm->nslots > kMaxSlots || tm->cur == nullptr)
+      return ast_fail(AST_ERR_ARG, "ast_step_f32: %d dt slots", tm->nslots);
+    kt = KTime{tm->prev, tm->cur, tm->rst, tm->nslots};
+  }
   if (!strip_fits(*g, krows))
     return ast_fail(AST_ERR_SIZE, "ast_step_f32: %d columns x %d rows per strip exceed one buffer range",
                     g->N, krows + 4);
@@ -146,22 +157,22 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   float2* S2 = (float2*)s2;
   float2* U2 = (float2*)u2;
   switch (method) {
-    case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
-    case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+    case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
+    case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
     case AST_EXACT:  // the learned start only where it can run (no window: full waves)
-      if (mlp != nullptr && !window && gate != nullptr) {
+      if (mlp != nullptr && !window && tm != nullptr && tm->gate != nullptr) {
         ExactCtx e0 = ex, e1 = ex;
-        e0.gate = e1.gate = gate;
-        e0.strong = e1.strong = gate + 1;
+        e0.gate = e1.gate = tm->prev ? reinterpret_cast<const uint32_t*>(tm->prev + 1) : tm->gate;
+        e0.strong = e1.strong = reinterpret_cast<uint32_t*>(tm->cur + 1);
         e0.gate_mode = 0;
         e1.gate_mode = 1;
-        rc = dispatch_geom<kExactDetect>(*g, c, e0, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+        rc = dispatch_geom<kExactDetect>(*g, c, e0, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
         if (rc != AST_OK) return rc;
-        return dispatch_geom<kExactLearned>(*g, c, e1, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+        return dispatch_geom<kExactLearned>(*g, c, e1, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
       }
       if (mlp != nullptr && !window)
-        return dispatch_geom<kExactLearned>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
-      return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, stats, krows, nslots, st);
+        return dispatch_geom<kExactLearned>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
+      return dispatch_geom<AST_EXACT>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
     default: return ast_fail(AST_ERR_ARG, "ast_step_f32: unknown method %d", method);
   }
 }
@@ -170,5 +181,5 @@ extern "C" int ast_step_f32(const ast_grid_t* g, const float* s, const float* u,
                             float* u2, double* T, const double* T_do, double* TT, int method,
                             const ast_mlp_t* mlp, ast_stats_t* stats, void* stream) {
   (void)TT;
-  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, 0, nullptr, stream);
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, nullptr, stream);
 }

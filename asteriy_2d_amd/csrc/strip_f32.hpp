@@ -51,9 +51,8 @@ AST_DEV Q shr1_keep(const Q& q) {
 }
 AST_DEV Q sub(const Q& a, const Q& b) { return Q{a.r - b.r, a.p - b.p, a.vx - b.vx, a.vy - b.vy}; }
 
-// half the minmod slope, 0.5*minmod(a, b) (kernel.cu:22-33): minmod(a, b) is
-// the median of {a, b, 0} -- one v_med3_f32.
-AST_DEV float hmm(float a, float b) { return 0.5f * __builtin_amdgcn_fmed3f(a, b, 0.f); }
+// minmod(a, b) (kernel.cu:22-33): the median of {a, b, 0} -- one v_med3_f32.
+AST_DEV float minmod_f(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
 
 // Face values at +-1/2 of a cell from its neighbour differences dl = c - a,
 // dr = b - c.  The reference's positivity fallback (kernel.cu:2706-2788:
@@ -61,11 +60,14 @@ AST_DEV float hmm(float a, float b) { return 0.5f * __builtin_amdgcn_fmed3f(a, b
 // 0 <= |mm| <= |dl|, |dr| of the same sign, c + mm/2 and c - mm/2 lie between
 // c/2 and the neighbours' means (also after rounding), so the fast path
 // requires rho, p > 0 in every cell instead of testing each face value.
+// c +- slope/2 as one fma each (0.5*slope is exact, so this rounds as
+// c +- (0.5*slope) does).
 AST_DEV void face_values(const Q& c, const Q& dl, const Q& dr, Q& plus, Q& minus) {
-  const float hr = hmm(dl.r, dr.r), hp = hmm(dl.p, dr.p);
-  const float hx = hmm(dl.vx, dr.vx), hy = hmm(dl.vy, dr.vy);
-  plus = Q{c.r + hr, c.p + hp, c.vx + hx, c.vy + hy};
-  minus = Q{c.r - hr, c.p - hp, c.vx - hx, c.vy - hy};
+  const float mr = minmod_f(dl.r, dr.r), mp = minmod_f(dl.p, dr.p);
+  const float mx = minmod_f(dl.vx, dr.vx), my = minmod_f(dl.vy, dr.vy);
+  plus = Q{fmaf(0.5f, mr, c.r), fmaf(0.5f, mp, c.p), fmaf(0.5f, mx, c.vx), fmaf(0.5f, my, c.vy)};
+  minus = Q{fmaf(-0.5f, mr, c.r), fmaf(-0.5f, mp, c.p), fmaf(-0.5f, mx, c.vx),
+            fmaf(-0.5f, my, c.vy)};
 }
 
 // Cell (n, m) in FP32 with the ghost rules; corner/out-of-halo positions are

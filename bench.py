@@ -4,8 +4,9 @@
 Headline workload: the stellar-wind / counter-flow interaction on a 2048x1024
 (z,r) grid, z in [-400,400], r in [0,400], gamma 5/3, HLLC + minmod, CFL 0.3
 (asteriy_2d_amd/problems.py).  A "step" is one pass of the step kernel over
-the grid followed by the time advance (the reference's Kernel_TVD +
-funk_time), replayed from a captured HIP graph with the state resident in HBM.
+the grid plus the time advance (the reference's Kernel_TVD + funk_time; the
+FP32 step kernel folds the previous step's dt and advances the time itself),
+replayed from captured HIP graphs of 20 steps with the state resident in HBM.
 
 `value`: Gcell-updates/s of the FP32 fast step over the K timed steps.  The
 same line carries the plain FP64 step (the bit-faithful restatement of the
@@ -18,7 +19,8 @@ the headline kernel; and the CPU baseline.
 
 Roofline: algorithmic bytes of the step kernel (FP32: 16 B read + 16 B
 written per cell-update) over its average launch duration, timed with HIP
-events on the stream the kernel runs on; peak 8 TB/s HBM.  CPU baseline: the
+events on the simulation's stream over back-to-back launches of the kernel
+as the runtime launches it; peak 8 TB/s HBM.  CPU baseline: the
 oracle (line-faithful C restatement of the reference step, FP64, OpenMP) on
 the host cores, on a bounded sample of full-grid steps.
 Progress goes to stderr; stdout is the one JSON line.
@@ -241,6 +243,8 @@ def main(argv=None):
                     help="only launch one step kernel --steps times (eager, for rocprofv3): the "
                          "HLLC step in FP32 / FP64, or --steps exact-solver steps of the wind "
                          "workload from its initial state")
+    ap.add_argument("--rows", type=int, default=0,
+                    help="--profile: FP32 strip height (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer --steps times (counter calibration)")
     a = ap.parse_args(argv)
@@ -279,11 +283,16 @@ def main(argv=None):
         sim.close()
         return 0
     if a.profile:
-        rows = 8
-        ms = _kernel_ms(g, a.profile, HLLC, torch.cuda.current_stream(), krows=rows, reps=K)
-        byt = (32.0 if a.profile == "f32" else 64.0) * cells
-        log(f"step_{a.profile} HLLC {N_GRID}x{M_GRID}: {ms * 1e3:.2f} us/launch, "
-            f"{byt / (ms * 1e-3) / 1e9:.1f} GB/s algorithmic")
+        # the HLLC steps as the headline runs them: autotuned strip height,
+        # graph replays (FP32: one step kernel per step + one closing fold)
+        sim = Simulation(g, HLLC, a.profile, p.s, p.u)
+        if a.rows:
+            sim.set_tile_rows(a.rows)
+        else:
+            sim.autotune()
+        sim.run(K).sync()
+        log(f"step_{a.profile} HLLC {N_GRID}x{M_GRID}: {K} steps")
+        sim.close()
         return 0
 
     # -- FP32 fast step (headline) ------------------------------------------
@@ -296,7 +305,9 @@ def main(argv=None):
     ms32 = _timed(sim32, K, st32)
     r32 = sim32.state()
     t32 = sim32.time()
-    kms32 = _kernel_ms(g, "f32", HLLC, st32, krows=rows)
+    # the step kernel alone, as the runtime launches it (dt slots), between
+    # HIP events on the simulation's stream over back-to-back launches
+    kms32 = sim32.autotune([rows], reps=200)[1] * 1e-3
     sim32.close()
     log(f"FP32 HLLC: {cells * K / ms32 / 1e6:.1f} Gcell/s, kernel {kms32 * 1e3:.1f} us, rows {rows}")
 

@@ -174,13 +174,19 @@ def test_f32_deterministic(dev):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
 
 
+@pytest.mark.parametrize("runs", [[(41, True)], [(7, True), (21, False), (1, True), (12, True)]])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_sim_graph_equals_eager(dev, precision):
+def test_sim_graph_equals_eager(dev, precision, runs):
+    """The runtime's steps (graph replays of 20 steps and direct launches; in
+    FP32 each step folds the previous step's dt slots and advances the time
+    itself) equal the reference protocol, Kernel_TVD + funk_time per step,
+    bit for bit, whatever the split of the steps into runs."""
     import torch
     from asteriy_2d_amd.sim import Simulation
     p = problems.wind(256, 128)
     with Simulation(p.grid, G.HLLC, precision, p.s, p.u) as a:
-        a.run(41, graph=True)
+        for n, graph in runs:
+            a.run(n, graph=graph)
         sa, ua = a.state()
         ta = a.time()
     dt = torch.float32 if precision == "f32" else torch.float64
@@ -213,17 +219,18 @@ def test_sim_counting_kernels_change_nothing(dev, precision):
     assert cb["capped"] == 0
 
 
-def test_sim_checkpoint_resume(dev, tmp_path):
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_sim_checkpoint_resume(dev, tmp_path, precision):
     from asteriy_2d_amd.sim import Simulation
     p = problems.wind(256, 128)
-    with Simulation(p.grid, G.HLLC, "f64", p.s, p.u) as a:
+    with Simulation(p.grid, G.HLLC, precision, p.s, p.u) as a:
         a.run(100)
         ref = a.state()
         tref = a.time()
-    with Simulation(p.grid, G.HLLC, "f64", p.s, p.u) as b:
+    with Simulation(p.grid, G.HLLC, precision, p.s, p.u) as b:
         b.run(50)
         b.save(str(tmp_path / "ck.bin"))
-    with Simulation(p.grid, G.HLLC, "f64") as c:
+    with Simulation(p.grid, G.HLLC, precision) as c:
         c.load(str(tmp_path / "ck.bin"))
         c.run(50)
         got = c.state()

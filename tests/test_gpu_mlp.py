@@ -188,7 +188,7 @@ def test_gated_learned_step_serves_strong_faces(dev):
             if mlp is not None:
                 sim.set_mlp(mlp)
             sim.collect_stats(True)
-            sim.run(10, graph=graph)
+            sim.run(25, graph=graph)  # graph: one replay of 20 steps + 5 direct
             res[tag] = (sim.state(), sim.stats())
     (a, ca), (b, cb), (c, _) = res["plain"], res["learned"], res["learned_eager"]
     assert ca["mlp_faces"] == 0 and cb["mlp_faces"] > 0, (ca, cb)
