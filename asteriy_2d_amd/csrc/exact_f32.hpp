@@ -309,12 +309,15 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   const float Urz = -e.c2g1 * cII * (1.f - q);
   const float Uvk = -e.c2g1 * (cII + cI);
   const float Udf = enI - enII;
+  // strong two-shock face: PVRS estimate above kStrongShock * pII (the
+  // learned kernel needs it before the branches for its wave ballot, the
+  // detecting one only in the two-shock branch)
   bool strong = false;
-  if constexpr (LEARNED || DETECT) {
+  auto strong_pvrs = [&]() {
     const float zI = rI * cI, zII = rII * cII;
-    const float pvrs = (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII);
-    strong = (Udf > Uud) && (pvrs > kStrongShock * pII);
-  }
+    return (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII) > kStrongShock * pII;
+  };
+  if constexpr (LEARNED) strong = (Udf > Uud) && strong_pvrs();
   if (LEARNED && e.mlp != nullptr) {
     if (__builtin_amdgcn_ballot_w64(strong) != 0) {  // wave-uniform
       float fe[mlp::IN];
@@ -356,6 +359,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
                            it, capped);
   } else {
     pat = 2;
+    if constexpr (DETECT && !LEARNED) strong = strong_pvrs();
     learned = guess > 0.f && guess < 3.0e38f;
     p = newton_f32(e, enI, pI, rI, cI, enII, pII, rII, cII, guess, learned, it, capped);
   }

@@ -49,13 +49,26 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   // gated launch: this kernel's mode is not the selected one (grid-uniform)
   if (ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
   const double tdo = step_dt(T, T_do, kt);
-  const int wg = xcd_tile(blockIdx.x, nwg);
   constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
   typename std::conditional<STATS, StatsF32,
                             typename std::conditional<kDetects, StrongStats, NoStats>::type>::type st;
   st.clear();
-  const double tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo,
-                                                          wg % nbands, wg / nbands, krows, st);
+  double tmin = INFINITY;
+  if constexpr (METHOD == kExactLearned) {
+    // several strip groups per workgroup when the grid is smaller than the
+    // work: the learned kernel runs on its co-resident capacity, so that it
+    // costs little in the steps in which it only reads the gate (the loop
+    // costs the other kernels registers, so they take one group each)
+    for (int item = blockIdx.x; item < nwg; item += gridDim.x) {  // workgroup-uniform
+      const int wg = xcd_tile(item, nwg);
+      tmin = fmin(tmin, strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo,
+                                                            wg % nbands, wg / nbands, krows, st));
+    }
+  } else {
+    const int wg = xcd_tile(blockIdx.x, nwg);
+    tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, wg % nbands,
+                                               wg / nbands, krows, st);
+  }
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
@@ -85,13 +98,17 @@ static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, 
   const int nsg = (nstrips + WPB - 1) / WPB;
   const long long nwg = (long long)nbands * nsg;
   if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
+  // the gated learned kernel: 512 workgroups (two waves per SIMD, what its
+  // registers allow to be resident at once) that loop over the strip groups
+  const unsigned grid =
+      (unsigned)(METHOD == kExactLearned && ex.gate != nullptr && nwg > 512 ? 512 : nwg);
   if (METHOD >= AST_EXACT && stats != nullptr) {
     if constexpr (METHOD >= AST_EXACT)
-      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3((unsigned)nwg), dim3(NT), 0,
+      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3(grid), dim3(NT), 0,
                          stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
                          (int)nwg);
   } else {
-    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false>), dim3((unsigned)nwg), dim3(NT), 0,
+    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false>), dim3(grid), dim3(NT), 0,
                        stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
                        (int)nwg);
   }

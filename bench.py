@@ -239,10 +239,11 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
-    ap.add_argument("--profile", choices=["f32", "f64", "exact32", "exact64"], default=None,
-                    help="only launch one step kernel --steps times (eager, for rocprofv3): the "
-                         "HLLC step in FP32 / FP64, or --steps exact-solver steps of the wind "
-                         "workload from its initial state")
+    ap.add_argument("--profile", choices=["f32", "f64", "exact32", "exact32l", "exact64"],
+                    default=None,
+                    help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
+                         "step in FP32 / FP64 from the initial state, or the exact-solver step "
+                         "(FP32, FP32 with the learned guess, FP64)")
     ap.add_argument("--rows", type=int, default=0,
                     help="--profile: FP32 strip height (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -275,9 +276,18 @@ def main(argv=None):
         torch.cuda.synchronize()
         log(f"copied {K} x {src.numel() * 4} bytes")
         return 0
-    if a.profile in ("exact32", "exact64"):
-        sim = Simulation(g, EXACT, "f32" if a.profile == "exact32" else "f64", p.s, p.u)
-        sim.run(K, graph=False)
+    if a.profile in ("exact32", "exact32l", "exact64"):
+        f32 = a.profile != "exact64"
+        sim = Simulation(g, EXACT, "f32" if f32 else "f64", p.s, p.u)
+        if a.profile == "exact32l":
+            from asteriy_2d_amd import learned
+            sim.set_mlp(learned.LearnedGuess.load())
+        if f32:
+            if a.rows:
+                sim.set_tile_rows(a.rows)
+            else:
+                sim.autotune()
+        sim.run(K)
         sim.sync()
         log(f"{a.profile}: {K} steps")
         sim.close()
@@ -349,9 +359,10 @@ def main(argv=None):
         pe = problems.wind(N_GRID, M_GRID, method=EXACT, steps=Ke)
         from asteriy_2d_amd import learned
         mdl = learned.LearnedGuess.load()
-        me32, e32, _, sx, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, autotune=True, stats=True)
-        mel, el, _, sxl, _ = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, mlp=mdl, autotune=True,
-                                      stats=True)
+        me32, e32, _, sx, rows_e = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, autotune=True,
+                                            stats=True)
+        mel, el, _, sxl, rows_l = _run_sim(g, EXACT, "f32", pe.s, pe.u, Ke, mlp=mdl,
+                                           autotune=True, stats=True)
         me64, e64, _, s64x, _ = _run_sim(g, EXACT, "f64", pe.s, pe.u, Ke, stats=True)
         extra["exact_2048x1024"] = {
             "steps": Ke,
@@ -363,6 +374,7 @@ def main(argv=None):
             "fp32_learned_speedup": round(me64 / mel, 3),
             "fp32_learned_error_vs_fp64": _errors(el, e64),
             "learned_faces": sxl["mlp_faces"],
+            "tile_rows": {"fp32": rows_e, "fp32_learned": rows_l},
             "error_vs_fp64": _errors(e32, e64),
             "newton_iters_per_face_fp32": round(sx["newton_iters"] / max(1, sx["newton_faces"]), 3),
             "newton_iters_per_face_fp64": round(s64x["newton_iters"] / max(1, s64x["newton_faces"]), 3),
@@ -441,7 +453,7 @@ def main(argv=None):
                    "grid": [N_GRID, M_GRID], "method": "hllc", "cfl": 0.3, "gamma": 5 / 3},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": _traffic("step_f32_hllc_2048x1024")},
+                     "traffic": _traffic(f"step_f32_hllc_2048x1024_r{rows}")},
         "cpu_baseline": cpu,
         **extra,
     }
