@@ -2,7 +2,7 @@
 (z,r) finite-volume step -- HIP kernels behind a C-ABI (include/asteriy.h),
 with a thin Python host mirror of the reference's operator interface."""
 from .grid import (AXISYMMETRIC, BC_DIRICHLET, BC_OUTFLOW, BC_OUTFLOW_CLAMPED, BC_REFLECT,
-                   EXACT, HLL, HLLC, PLANAR, Grid, reference_live_grid)
+                   EXACT, HLL, HLLC, HLLC_A, PLANAR, Grid, reference_live_grid)
 
-__all__ = ["Grid", "reference_live_grid", "HLL", "HLLC", "EXACT", "PLANAR", "AXISYMMETRIC",
+__all__ = ["Grid", "reference_live_grid", "HLL", "HLLC", "EXACT", "HLLC_A", "PLANAR", "AXISYMMETRIC",
            "BC_OUTFLOW", "BC_REFLECT", "BC_DIRICHLET", "BC_OUTFLOW_CLAMPED"]

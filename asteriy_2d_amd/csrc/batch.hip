@@ -144,6 +144,7 @@ static int batch_step(ast_batch_t* B, int from, bool advance_time) {
   switch (B->method) {
     case AST_HLL: rc = launch_batch<AST_HLL>(B, from); break;
     case AST_HLLC: rc = launch_batch<AST_HLLC>(B, from); break;
+    case AST_HLLC_A: rc = launch_batch<AST_HLLC_A>(B, from); break;
     default: rc = launch_batch<AST_EXACT>(B, from); break;
   }
   if (rc != AST_OK || !advance_time) return rc;
@@ -156,7 +157,7 @@ static int batch_step(ast_batch_t* B, int from, bool advance_time) {
 extern "C" int ast_batch_create(int count, const ast_grid_t* grids, int method, ast_batch_t** out) {
   if (!out || !grids || count < 1) return ast_fail(AST_ERR_ARG, "ast_batch_create: bad argument");
   *out = nullptr;
-  if (method < AST_HLL || method > AST_EXACT)
+  if (method < AST_HLL || method > AST_HLLC_A)
     return ast_fail(AST_ERR_ARG, "ast_batch_create: method %d", method);
   size_t floats = 0;
   for (int k = 0; k < count; k++) {
@@ -217,6 +218,7 @@ extern "C" int ast_batch_create(int count, const ast_grid_t* grids, int method, 
   switch (method) {
     case AST_HLL: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<AST_HLL>, NT, 0); break;
     case AST_HLLC: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<AST_HLLC>, NT, 0); break;
+    case AST_HLLC_A: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<AST_HLLC_A>, NT, 0); break;
     default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<AST_EXACT>, NT, 0); break;
   }
   if (oe != hipSuccess || per_cu < 1) per_cu = 1;

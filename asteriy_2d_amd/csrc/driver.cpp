@@ -9,7 +9,7 @@
 // on request, its text outputs (Tecplot file of every 2nd node, the axis row,
 // kernel.cu:4127-4215).
 //
-//   asteriy_run --problem wind|sod|live [--N n --M m] [--method hll|hllc|exact]
+//   asteriy_run --problem wind|sod|live [--N n --M m] [--method hll|hllc|exact|hllc_a]
 //               [--precision f64|f32] [--steps k] [--print-every k]
 //               [--out B31.bin] [--tecplot file] [--axis file]
 //               [--checkpoint file] [--resume file] [--mlp params.f32] [--autotune]
@@ -41,7 +41,7 @@ int die(const char* what) {
 
 void usage() {
   printf(
-      "usage: asteriy_run --problem wind|sod|live [--N n --M m] [--method hll|hllc|exact]\n"
+      "usage: asteriy_run --problem wind|sod|live [--N n --M m] [--method hll|hllc|exact|hllc_a]\n"
       "                   [--precision f64|f32] [--steps k] [--print-every k] [--out B31.bin]\n"
       "                   [--tecplot file] [--axis file] [--checkpoint file] [--resume file]\n"
       "                   [--mlp params.f32] [--autotune]\n"
@@ -180,7 +180,7 @@ int main(int argc, char** argv) {
     else { fprintf(stderr, "asteriy_run: unknown option %s\n", a.c_str()); usage(); return 2; }
   }
   const int method = o.method == "hll" ? AST_HLL : o.method == "hllc" ? AST_HLLC
-                     : o.method == "exact" ? AST_EXACT : -1;
+                     : o.method == "exact" ? AST_EXACT : o.method == "hllc_a" ? AST_HLLC_A : -1;
   const int prec = o.precision == "f64" ? 0 : o.precision == "f32" ? 1 : -1;
   if (method < 0 || prec < 0 || o.steps < 0 || o.every < 1) { usage(); return 2; }
   ast_grid_t g;

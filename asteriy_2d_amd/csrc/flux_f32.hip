@@ -1,5 +1,6 @@
 // flux_f32.hip -- batched FP32 face-flux operators over SoA face arrays:
-// HLL, HLLC and the exact solver (FP64 islands) of the fast path applied to
+// HLL, HLLC (both wave-speed estimates) and the exact solver (FP64 islands)
+// of the fast path applied to
 // n independent faces with a common normal (n1, n2).  52 B/face of
 // algorithmic traffic (8 words in, 5 out).
 #include <type_traits>
@@ -42,7 +43,8 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
     float UU;
     FluxF f;
     if constexpr (METHOD == AST_HLL) f = hll_f32(L, R, ex.g, ex.ig1, UU);
-    else if constexpr (METHOD == AST_HLLC) f = hllc_f32(L, R, ex.g, ex.ig1, UU);
+    else if constexpr (METHOD == AST_HLLC) f = hllc_f32<false>(L, R, ex.g, ex.ig1, UU);
+    else if constexpr (METHOD == AST_HLLC_A) f = hllc_f32<true>(L, R, ex.g, ex.ig1, UU);
     else {
       FaceInfo fi;
       f = exact_f32<LEARNED>(L, R, ex, UU, fi);
@@ -107,6 +109,7 @@ extern "C" int ast_flux_f32_info(int method, size_t n, const float* in, float n1
     case AST_HLL: return launch<AST_HLL>(n, in, n1, n2, krad, ex, out, stats, nullptr, st);
     case AST_HLLC: return launch<AST_HLLC>(n, in, n1, n2, krad, ex, out, stats, nullptr, st);
     case AST_EXACT: return launch<AST_EXACT>(n, in, n1, n2, krad, ex, out, stats, info, st);
+    case AST_HLLC_A: return launch<AST_HLLC_A>(n, in, n1, n2, krad, ex, out, stats, nullptr, st);
     default: return ast_fail(AST_ERR_ARG, "ast_flux_f32: unknown method %d", method);
   }
 }

@@ -1,4 +1,4 @@
-// flux_f64.hip -- batched FP64 face-flux operators (HLL, HLLC, exact) over
+// flux_f64.hip -- batched FP64 face-flux operators (HLL, HLLC, exact, HLLC_A) over
 // SoA face arrays: the reference's per-face operators (kernel.cu:1399,
 // Gas_Dinamic.cu:1814, kernel.cu:128) applied to n independent faces.
 // Built with -ffp-contract=off for bit parity with the reference arithmetic.
@@ -31,6 +31,7 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const double* __res
     double t;
     if constexpr (METHOD == AST_HLL) t = HLL(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
     else if constexpr (METHOD == AST_HLLC) t = HLLC(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    else if constexpr (METHOD == AST_HLLC_A) t = HLLC_A(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
     else t = GODUNOV(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad, st);
     out[i] = Ps.x;
     out[n + i] = Ps.y;
@@ -122,6 +123,7 @@ extern "C" int ast_flux_f64(int method, size_t n, const double* in, double n1, d
     case AST_HLL: return launch_flux<AST_HLL>(n, in, n1, n2, rad, ph, out, stats, st);
     case AST_HLLC: return launch_flux<AST_HLLC>(n, in, n1, n2, rad, ph, out, stats, st);
     case AST_EXACT: return launch_flux<AST_EXACT>(n, in, n1, n2, rad, ph, out, stats, st);
+    case AST_HLLC_A: return launch_flux<AST_HLLC_A>(n, in, n1, n2, rad, ph, out, stats, st);
     default: return ast_fail(AST_ERR_ARG, "ast_flux_f64: unknown method %d", method);
   }
 }

@@ -1,7 +1,7 @@
 // ref_flux.hpp -- device face-flux operators in the reference's own form.
 //
-// HLL (kernel.cu:1399-1523), HLLC_2d_Korolkov (Gas_Dinamic.cu:1604-1826) and
-// the exact Godunov_Solver_Alexashov with newton/devtwo/lev
+// HLL (kernel.cu:1399-1523), HLLC_2d_Korolkov (Gas_Dinamic.cu:1604-1826),
+// HLLC_Aleksashov (kernel.cu:767-1079) and the exact Godunov_Solver_Alexashov with newton/devtwo/lev
 // (kernel.cu:128-765), written for the GPU with the reference's formulas,
 // constants, branch structure and operation order, so that a translation
 // unit compiled with -ffp-contract=off reproduces the reference arithmetic
@@ -235,6 +235,77 @@ AST_DEV R HLLC(const Phys<R>& ph, const V2<R>& Ls, const V2<R>& Lu, const V2<R>&
   Ps.y = P3;
   Pu.x = P1;
   Pu.y = P2;
+  return time;
+}
+
+// HLLC_Aleksashov (kernel.cu:767-1079) as add_MK calls it: zero magnetic
+// field and out-of-plane velocity, so every term carrying them is an exact
+// zero and is left out -- the frame's else branch (kernel.cu:819-861) gives
+// the in-plane tangent (-n2, n1), the fast speeds reduce to
+// sqrt(c*(c + 2*0)), pt = p, the HLL state feeds only B terms and the star
+// tangential velocities are the sides' own (kernel.cu:1009-1019).  Wave
+// speeds min(vL - cL, vR - cR), max(vL + cL, vR + cR) (kernel.cu:895-896),
+// averaged ptz in the star energy (kernel.cu:981-1007), wv = 0.
+template <typename R>
+AST_DEV R HLLC_A(const Phys<R>& ph, const V2<R>& Ls, const V2<R>& Lu, const V2<R>& Rs,
+                 const V2<R>& Ru, R n1, R n2, V2<R>& Ps, V2<R>& Pu, R rad) {
+  typedef Mth<R> M;
+  const R ga = ph.ga, g1 = ph.g1;
+  const R r1 = Ls.x, p1 = Ls.y, u1 = Lu.x, v1 = Lu.y;
+  const R r2 = Rs.x, p2 = Rs.y, u2 = Ru.x, v2 = Ru.y;
+  const R vL0 = n1 * u1 + n2 * v1, vL2 = -n2 * u1 + n1 * v1;  // kernel.cu:861-867
+  const R vR0 = n1 * u2 + n2 * v2, vR2 = -n2 * u2 + n1 * v2;
+  const R cL = M::sqrt_(ga * p1 / r1);  // kernel.cu:869-891
+  R qp = M::sqrt_(cL * (cL + R(2.0) * R(0.0))), qm = M::sqrt_(cL * (cL - R(2.0) * R(0.0)));
+  const R cfL = (qp + qm) / R(2.0);
+  const R cR = M::sqrt_(ga * p2 / r2);
+  qp = M::sqrt_(cR * (cR + R(2.0) * R(0.0)));
+  qm = M::sqrt_(cR * (cR - R(2.0) * R(0.0)));
+  const R cfR = (qp + qm) / R(2.0);
+  const R ptL = p1, ptR = p2;
+  const R SL = M::fmin_(vL0 - cfL, vR0 - cfR);
+  const R SR = M::fmax_(vL0 + cfL, vR0 + cfR);
+  const R suR = SR - vR0;
+  const R suL = SL - vL0;
+  const R SM = (suR * r2 * vR0 - ptR + ptL - suL * r1 * vL0) / (suR * r2 - suL * r1);
+  const R UU = M::fmax_(M::fabs_(SL), M::fabs_(SR));
+  const R time = ph.krit * rad / UU;
+  const R upt1 = (u1 * u1 + v1 * v1) / R(2.0);  // kernel.cu:928-975
+  const R upt2 = (u2 * u2 + v2 * v2) / R(2.0);
+  const R e1 = p1 / g1 + r1 * upt1;
+  const R e2 = p2 / g1 + r2 * upt2;
+  // mass, normal and tangential momentum, energy
+  const R FL[4] = {r1 * vL0, r1 * vL0 * vL0 + ptL, r1 * vL0 * vL2, (e1 + ptL) * vL0};
+  const R FR[4] = {r2 * vR0, r2 * vR0 * vR0 + ptR, r2 * vR0 * vR2, (e2 + ptR) * vR0};
+  const R UL[4] = {r1, r1 * vL0, r1 * vL2, e1};
+  const R UR[4] = {r2, r2 * vR0, r2 * vR2, e2};
+  const R suRm = suR / (SR - SM);  // kernel.cu:977-1031
+  const R suLm = suL / (SL - SM);
+  const R rzR = r2 * suRm;
+  const R rzL = r1 * suLm;
+  const R ptzR = ptR + r2 * suR * (SM - vR0);
+  const R ptzL = ptL + r1 * suL * (SM - vL0);
+  const R ptz = (ptzR + ptzL) / R(2.0);
+  const R ezR = e2 * suRm + (ptz * SM - ptR * vR0) / (SR - SM);
+  const R ezL = e1 * suLm + (ptz * SM - ptL * vL0) / (SL - SM);
+  const R UZL[4] = {rzL, SM * rzL, vL2 * rzL, ezL};
+  const R UZR[4] = {rzR, SM * rzR, vR2 * rzR, ezR};
+  R P[4];
+  if (SL > R(0.0)) {  // kernel.cu:1033-1068
+    for (int k = 0; k < 4; k++) P[k] = FL[k];
+  } else if (SL <= R(0.0) && SM >= R(0.0)) {
+    for (int k = 0; k < 4; k++) P[k] = FL[k] + SL * (UZL[k] - UL[k]);
+  } else if (SM <= R(0.0) && SR >= R(0.0)) {
+    for (int k = 0; k < 4; k++) P[k] = FR[k] + SR * (UZR[k] - UR[k]);
+  } else if (SR < R(0.0)) {
+    for (int k = 0; k < 4; k++) P[k] = FR[k];
+  } else {  // NaN speeds
+    for (int k = 0; k < 4; k++) P[k] = M::nan_();
+  }
+  Ps.x = P[0];
+  Ps.y = P[3];
+  Pu.x = n1 * P[1] - n2 * P[2];  // kernel.cu:1075-1076
+  Pu.y = n2 * P[1] + n1 * P[2];
   return time;
 }
 

@@ -126,7 +126,7 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
   if (rc != AST_OK) return rc;
   if (precision != 0 && precision != 1)
     return ast_fail(AST_ERR_ARG, "ast_sim_create: precision %d (0 = FP64, 1 = FP32)", precision);
-  if (method < AST_HLL || method > AST_EXACT)
+  if (method < AST_HLL || method > AST_HLLC_A)
     return ast_fail(AST_ERR_ARG, "ast_sim_create: method %d", method);
   if (ast_device_count() < 1) return ast_fail(AST_ERR_NOGPU, "ast_sim_create: no HIP device");
   ast_sim_t* S = new (std::nothrow) ast_sim_t();

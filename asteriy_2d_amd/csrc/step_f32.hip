@@ -102,8 +102,8 @@ static int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, 
   // registers allow to be resident at once) that loop over the strip groups
   const unsigned grid =
       (unsigned)(METHOD == kExactLearned && ex.gate != nullptr && nwg > 512 ? 512 : nwg);
-  if (METHOD >= AST_EXACT && stats != nullptr) {
-    if constexpr (METHOD >= AST_EXACT)
+  if (is_exact(METHOD) && stats != nullptr) {
+    if constexpr (is_exact(METHOD))
       hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3(grid), dim3(NT), 0,
                          stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
                          (int)nwg);
@@ -176,6 +176,7 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   switch (method) {
     case AST_HLL: return dispatch_geom<AST_HLL>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
     case AST_HLLC: return dispatch_geom<AST_HLLC>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
+    case AST_HLLC_A: return dispatch_geom<AST_HLLC_A>(*g, c, ex, window, S, U, S2, U2, T, T_do, kt, stats, krows, st);
     case AST_EXACT:  // the learned start only where it can run (no window: full waves)
       if (mlp != nullptr && !window && tm != nullptr && tm->gate != nullptr) {
         ExactCtx e0 = ex, e1 = ex;

@@ -30,6 +30,8 @@ AST_DEV double face_flux(const Phys<double>& ph, const D2& Ls, const D2& Lu, con
     return HLL(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
   } else if constexpr (METHOD == AST_HLLC) {
     return HLLC(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+  } else if constexpr (METHOD == AST_HLLC_A) {
+    return HLLC_A(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
   } else {
     return GODUNOV(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad, st);
   }
@@ -271,8 +273,12 @@ extern "C" int ast_step_f64(const ast_grid_t* g, const double* s, const double* 
     case AST_EXACT:
       return axi ? launch<AST_EXACT, AST_AXISYMMETRIC>(*g, S, U, S2, U2, T, T_do, stats, st)
                  : launch<AST_EXACT, AST_PLANAR>(*g, S, U, S2, U2, T, T_do, stats, st);
+    case AST_HLLC_A:
+      return axi ? launch<AST_HLLC_A, AST_AXISYMMETRIC>(*g, S, U, S2, U2, T, T_do, stats, st)
+                 : launch<AST_HLLC_A, AST_PLANAR>(*g, S, U, S2, U2, T, T_do, stats, st);
     default:
-      return ast_fail(AST_ERR_ARG, "ast_step_f64: method %d is not AST_HLL/AST_HLLC/AST_EXACT",
+      return ast_fail(AST_ERR_ARG,
+                      "ast_step_f64: method %d is not AST_HLL/AST_HLLC/AST_EXACT/AST_HLLC_A",
                       method);
   }
 }

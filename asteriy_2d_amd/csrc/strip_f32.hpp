@@ -160,14 +160,19 @@ static inline int check_positive_bc(const ast_grid_t& g) {
 // Kernel-internal method ids: the exact solver with the learned newton start
 // compiled in (AST_EXACT kernels leave it out), and the plain exact solver
 // that reports strong two-shock faces (the other half of a gated launch).
-constexpr int kExactLearned = AST_EXACT + 1;
-constexpr int kExactDetect = AST_EXACT + 2;
+constexpr int kExactLearned = 16;
+constexpr int kExactDetect = 17;
+// the exact solver, in any of its kernel variants
+constexpr bool is_exact(int method) {
+  return method == AST_EXACT || method == kExactLearned || method == kExactDetect;
+}
 
 template <int METHOD>
 AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
                         const ExactCtx& ex, FaceInfo& fi) {
   if constexpr (METHOD == AST_HLL) return hll_f32(L, R, c.g, c.ig1, UU);
-  else if constexpr (METHOD == AST_HLLC) return hllc_f32(L, R, c.g, c.ig1, UU);
+  else if constexpr (METHOD == AST_HLLC) return hllc_f32<false>(L, R, c.g, c.ig1, UU);
+  else if constexpr (METHOD == AST_HLLC_A) return hllc_f32<true>(L, R, c.g, c.ig1, UU);
   else return exact_f32<METHOD == kExactLearned, METHOD == kExactDetect>(L, R, ex, UU, fi);
 }
 
@@ -202,7 +207,7 @@ AST_DEV FluxF face(const Q& l, const Q& r, const Q& lc, const Q& rc, bool ho, co
     R = NORMAL_X ? PrimF{r.r, r.p, r.vx, r.vy} : PrimF{r.r, r.p, r.vy, r.vx};
     f = face_flux<METHOD>(L, R, c, UU, ex, fi);
   }
-  if constexpr (METHOD >= AST_EXACT) st.add(fi);
+  if constexpr (is_exact(METHOD)) st.add(fi);
   return f;
 }
 

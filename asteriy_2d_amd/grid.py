@@ -12,8 +12,8 @@ import math
 from dataclasses import dataclass, field
 from typing import List, Sequence
 
-HLL, HLLC, EXACT = 0, 1, 2
-METHODS = {"hll": HLL, "hllc": HLLC, "exact": EXACT}
+HLL, HLLC, EXACT, HLLC_A = 0, 1, 2, 3
+METHODS = {"hll": HLL, "hllc": HLLC, "exact": EXACT, "hllc_a": HLLC_A}
 PLANAR, AXISYMMETRIC = 0, 1
 BC_OUTFLOW, BC_REFLECT, BC_DIRICHLET, BC_OUTFLOW_CLAMPED = 0, 1, 2, 3
 XLO, XHI, YLO, YHI = 0, 1, 2, 3

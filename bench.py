@@ -93,7 +93,8 @@ def _kernel_ms(grid, precision, method, stream, krows=None, reps=50):
 
 
 def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
-    """HLL, HLLC and the exact solver over n seeded face pairs (the
+    """HLL, HLLC (HLLC_2d_Korolkov and HLLC_Aleksashov) and the exact solver over
+    n seeded face pairs (the
     microbenchmark distribution, 2% near vacuum), FP64 and FP32 (the exact one
     also with the learned newton start): Gfaces/s over `reps` launches with the
     inputs resident, HBM fraction at 104 / 52 B per face, and the exact
@@ -102,7 +103,7 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
 
     import torch
 
-    from asteriy_2d_amd import EXACT, HLL, HLLC, facegen, native, solver
+    from asteriy_2d_amd import EXACT, HLL, HLLC, HLLC_A, facegen, native, solver
     dev = torch.device("cuda:0")
     F64 = torch.from_numpy(facegen.faces(n)).to(dev)
     F32 = F64.float()
@@ -128,7 +129,7 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
         return e0.elapsed_time(e1) / reps  # ms per launch
 
     res = {"faces": n}
-    for name, m in (("hll", HLL), ("hllc", HLLC), ("exact", EXACT)):
+    for name, m in (("hll", HLL), ("hllc", HLLC), ("hllc_a", HLLC_A), ("exact", EXACT)):
         t64 = timed(lambda: solver.flux_batch(m, F64, out=o64, stream=stream))
         t32 = timed(lambda: call32(m, None))
         r = {"f64_gfaces": round(n / t64 / 1e6, 3), "f32_gfaces": round(n / t32 / 1e6, 3),

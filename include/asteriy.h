@@ -37,8 +37,10 @@ enum {
 };
 
 /* ---- flux methods: the numbering of add_MK's `method` (kernel.cu:2385-2463),
- *      with method 1 bound to the live HLLC_2d_Korolkov (Gas_Dinamic.cu:1814). */
-enum { AST_HLL = 0, AST_HLLC = 1, AST_EXACT = 2 };
+ *      with method 1 bound to the live HLLC_2d_Korolkov (Gas_Dinamic.cu:1814);
+ *      AST_HLLC_A is HLLC_Aleksashov (kernel.cu:767-1079, add_MK's method 1)
+ *      with the zero magnetic field it is called with. */
+enum { AST_HLL = 0, AST_HLLC = 1, AST_EXACT = 2, AST_HLLC_A = 3 };
 
 /* ---- geometry of the conservative update ------------------------------- */
 enum {

@@ -26,6 +26,10 @@
  *                                 antisymmetry, HLL closed form.
  *  ora_hllc                     : consistency, upwinding, stationary contact,
  *                                 mirror antisymmetry to round-off.
+ *  ora_hllc_a                   : the same invariants, and equality with
+ *                                 ora_hllc to round-off where both sides have
+ *                                 the same sound speed (the two wave-speed
+ *                                 estimates then coincide).
  *  ora_godunov (+newton/devtwo) : independent Toro exact solver (toro.c),
  *                                 Toro's published test table, the
  *                                 reference's own `test` kernel KAT
@@ -288,6 +292,93 @@ out:
   Ps[1] = P[3];
   Pu[0] = P[1];
   Pu[1] = P[2];
+  return time;
+}
+
+/* HLLC_Aleksashov, kernel.cu:767-1079, as add_MK calls it: zero magnetic
+ * field (bx = by = bz = 0) and zero out-of-plane velocity (w = 0).  Every
+ * term that carries B or w is then an exact zero and is left out: the
+ * field-aligned frame takes its else branch (kernel.cu:819-861: first
+ * tangent the z axis, second (-n2, n1)), so the in-plane tangential velocity
+ * is the second tangent component; the Alfven terms vanish (aa = b2 = 0:
+ * qp = qm = sqrt(c*(c + 0)), kept as written), pt = p, the HLL state UZ only
+ * feeds B terms, and with |UZ[5]| = 0 < eps the star tangential velocities
+ * are the sides' own (kernel.cu:1009-1019).  The unused averaged-state speed
+ * cfC, the repeated speed evaluation when SM leaves [SL, SR] (kernel.cu:
+ * 911-926: it recomputes the same values) and the prints are left out.
+ * Wave speeds: SL = min(vL - cfL, vR - cfR), SR = max(vL + cfL, vR + cfR)
+ * (kernel.cu:895-896), star energy with the averaged ptz (kernel.cu:981-985,
+ * 1006-1007), frame speed wv = 0. */
+double ora_hllc_a(const ora_phys* ph, const double* Ls, const double* Lu,
+                  const double* Rs, const double* Ru, double n1, double n2,
+                  double* Ps, double* Pu, double rad) {
+  const double ga = ph->ga, g1 = ph->g1;
+  const double r1 = Ls[0], p1 = Ls[1], u1 = Lu[0], v1 = Lu[1];
+  const double r2 = Rs[0], p2 = Rs[1], u2 = Ru[0], v2 = Ru[1];
+  /* frame: normal (n1, n2), tangent (-n2, n1) (kernel.cu:861-867) */
+  const double vL0 = n1 * u1 + n2 * v1, vL2 = -n2 * u1 + n1 * v1;
+  const double vR0 = n1 * u2 + n2 * v2, vR2 = -n2 * u2 + n1 * v2;
+  /* fast speeds, kernel.cu:869-891 */
+  const double cL = sqrt(ga * p1 / r1);
+  double qp = sqrt(cL * (cL + 2.0 * 0.0)), qm = sqrt(cL * (cL - 2.0 * 0.0));
+  const double cfL = (qp + qm) / 2.0;
+  const double cR = sqrt(ga * p2 / r2);
+  qp = sqrt(cR * (cR + 2.0 * 0.0));
+  qm = sqrt(cR * (cR - 2.0 * 0.0));
+  const double cfR = (qp + qm) / 2.0;
+  const double ptL = p1, ptR = p2;
+  double SL = fmin(vL0 - cfL, vR0 - cfR);
+  double SR = fmax(vL0 + cfL, vR0 + cfR);
+  double suR = SR - vR0;
+  double suL = SL - vL0;
+  double SM = (suR * r2 * vR0 - ptR + ptL - suL * r1 * vL0) / (suR * r2 - suL * r1);
+  double UU = fmax(fabs(SL), fabs(SR));
+  double time = ph->krit * rad / UU;
+  /* kernel.cu:928-975 */
+  double upt1 = (u1 * u1 + v1 * v1) / 2.0;
+  double upt2 = (u2 * u2 + v2 * v2) / 2.0;
+  double e1 = p1 / g1 + r1 * upt1;
+  double e2 = p2 / g1 + r2 * upt2;
+  double FL[4], FR[4], UL[4], UR[4]; /* mass, normal and tangential momentum, energy */
+  FL[0] = r1 * vL0;
+  FL[1] = r1 * vL0 * vL0 + ptL;
+  FL[2] = r1 * vL0 * vL2;
+  FL[3] = (e1 + ptL) * vL0;
+  FR[0] = r2 * vR0;
+  FR[1] = r2 * vR0 * vR0 + ptR;
+  FR[2] = r2 * vR0 * vR2;
+  FR[3] = (e2 + ptR) * vR0;
+  UL[0] = r1; UL[1] = r1 * vL0; UL[2] = r1 * vL2; UL[3] = e1;
+  UR[0] = r2; UR[1] = r2 * vR0; UR[2] = r2 * vR2; UR[3] = e2;
+  /* star states, kernel.cu:977-1031 */
+  double suRm = suR / (SR - SM);
+  double suLm = suL / (SL - SM);
+  double rzR = r2 * suRm;
+  double rzL = r1 * suLm;
+  double ptzR = ptR + r2 * suR * (SM - vR0);
+  double ptzL = ptL + r1 * suL * (SM - vL0);
+  double ptz = (ptzR + ptzL) / 2.0;
+  double ezR = e2 * suRm + (ptz * SM - ptR * vR0) / (SR - SM);
+  double ezL = e1 * suLm + (ptz * SM - ptL * vL0) / (SL - SM);
+  double UZL[4] = {rzL, SM * rzL, vL2 * rzL, ezL};
+  double UZR[4] = {rzR, SM * rzR, vR2 * rzR, ezR};
+  double P[4];
+  if (SL > 0.0) {                        /* kernel.cu:1033-1041 */
+    for (int k = 0; k < 4; k++) P[k] = FL[k];
+  } else if (SL <= 0.0 && SM >= 0.0) {   /* kernel.cu:1042-1050 */
+    for (int k = 0; k < 4; k++) P[k] = FL[k] + SL * (UZL[k] - UL[k]);
+  } else if (SM <= 0.0 && SR >= 0.0) {   /* kernel.cu:1051-1059 */
+    for (int k = 0; k < 4; k++) P[k] = FR[k] + SR * (UZR[k] - UR[k]);
+  } else if (SR < 0.0) {                 /* kernel.cu:1060-1068 */
+    for (int k = 0; k < 4; k++) P[k] = FR[k];
+  } else {                               /* NaN speeds: the reference prints */
+    for (int k = 0; k < 4; k++) P[k] = NAN;
+  }
+  Ps[0] = P[0];
+  Ps[1] = P[3];
+  /* back to the lab frame, kernel.cu:1075-1076 */
+  Pu[0] = n1 * P[1] - n2 * P[2];
+  Pu[1] = n2 * P[1] + n1 * P[2];
   return time;
 }
 
@@ -606,6 +697,7 @@ double ora_flux(int method, const ora_phys* ph, const double* Ls,
   switch (method) {
     case AST_HLL: return ora_hll(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
     case AST_HLLC: return ora_hllc(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    case AST_HLLC_A: return ora_hllc_a(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
     default: return ora_godunov(ph, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
   }
 }
@@ -760,7 +852,8 @@ static double o_cell(const ast_grid_t* g, const ora_phys* ph, const d2* s,
   double tmin = 1000;
   double A = 0, B = 0;
   o_flux_fn F = method == AST_HLL ? ora_hll
-              : method == AST_HLLC ? ora_hllc : ora_godunov;
+              : method == AST_HLLC ? ora_hllc
+              : method == AST_HLLC_A ? ora_hllc_a : ora_godunov;
 
   if (o_frozen(g, x, y)) {  /* kernel.cu:2317-2324: copied, no dt vote */
     s2[index] = s_1;

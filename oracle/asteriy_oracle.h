@@ -31,6 +31,9 @@ double ora_hll(const ora_phys*, const double* Ls, const double* Lu,
 double ora_hllc(const ora_phys*, const double* Ls, const double* Lu,
                 const double* Rs, const double* Ru, double n1, double n2,
                 double* Ps, double* Pu, double rad);
+double ora_hllc_a(const ora_phys*, const double* Ls, const double* Lu,
+                const double* Rs, const double* Ru, double n1, double n2,
+                double* Ps, double* Pu, double rad);
 double ora_godunov(const ora_phys*, const double* Ls, const double* Lu,
                    const double* Rs, const double* Ru, double n1, double n2,
                    double* Ps, double* Pu, double rad);

@@ -50,7 +50,7 @@ def lib():
         L.ora_linear.restype = ctypes.c_double
         L.ora_linear.argtypes = [ctypes.c_double] * 7
         L.ora_linear2.argtypes = [ctypes.c_double] * 8 + [_dp, _dp]
-        for fn in ("ora_hll", "ora_hllc", "ora_godunov"):
+        for fn in ("ora_hll", "ora_hllc", "ora_hllc_a", "ora_godunov"):
             f = getattr(L, fn)
             f.restype = ctypes.c_double
             f.argtypes = [ctypes.POINTER(CPhys), _dp, _dp, _dp, _dp, ctypes.c_double,
@@ -105,7 +105,7 @@ def linear2(x1, t1, x2, t2, x3, t3, y1, y2):
     return A.value, B.value
 
 
-_FLUX = {0: "ora_hll", 1: "ora_hllc", 2: "ora_godunov"}
+_FLUX = {0: "ora_hll", 1: "ora_hllc", 2: "ora_godunov", 3: "ora_hllc_a"}
 
 
 def flux(method: int, Ls, Lu, Rs, Ru, n1, n2, rad, gamma=5.0 / 3.0, krit=0.05):

@@ -41,7 +41,7 @@ def _l1(a, b):
     return float(np.abs(a - b).sum() / np.abs(b).sum())
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_flux_f32_vs_f64(dev, method):
     import torch
     from asteriy_2d_amd import native, solver
@@ -62,7 +62,7 @@ def test_flux_f32_vs_f64(dev, method):
     q = [float(np.quantile(rel, x)) for x in (0.5, 0.99, 0.999)]
     print(f"method {method}: median {q[0]:.2e} p99 {q[1]:.2e} p99.9 {q[2]:.2e}")
     tol = {G.HLL: (1e-7, 1e-5, 1e-4), G.HLLC: (1e-7, 1e-5, 1e-4),
-           G.EXACT: (1e-7, 1e-5, 1e-4)}[method]
+           G.EXACT: (1e-7, 1e-5, 1e-4), G.HLLC_A: (1e-7, 1e-5, 1e-4)}[method]
     assert q[0] < tol[0] and q[1] < tol[1] and q[2] < tol[2]
     # dt: the exact solver's largest wave speed depends on the wave pattern,
     # and faces on a pattern boundary may classify differently in FP32
@@ -105,7 +105,7 @@ def test_exact_f32_fp64_islands(dev):
         assert q[0] < 1e-5 and q[1] < 1e-4
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 @pytest.mark.parametrize("geometry", [G.PLANAR, G.AXISYMMETRIC])
 def test_step_f32_one_step_vs_f64(dev, method, geometry):
     import torch
@@ -119,7 +119,7 @@ def test_step_f32_one_step_vs_f64(dev, method, geometry):
     assert np.max(np.abs(a[1] - b[1])) < 1e-6
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_step_f32_discontinuous_vs_f64(dev, method):
     import torch
     g = G.Grid.from_extent(130, 70, -1.0, 2.0, 1.5, krit=0.3, geometry=G.AXISYMMETRIC,

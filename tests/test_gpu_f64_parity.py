@@ -30,7 +30,7 @@ def _close(a, b, rel=REL, scale=None):
     return int(bad.sum())
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_flux_f64_matches_oracle(dev, ora, method):
     import torch
     from asteriy_2d_amd import solver
@@ -40,7 +40,7 @@ def test_flux_f64_matches_oracle(dev, ora, method):
     out = solver.flux_batch(method, _t(faces, dev), 1.0, 0.0, 0.25).cpu().numpy()
     torch.cuda.synchronize()
     ref = ora.flux_batch(method, faces, 1.0, 0.0, 0.25)
-    if method in (G.HLL, G.HLLC):
+    if method in (G.HLL, G.HLLC, G.HLLC_A):
         same = (out == ref) | (np.isnan(out) & np.isnan(ref))
         assert same.all(), f"{(~same).sum()} values differ bitwise"
     else:
@@ -54,7 +54,7 @@ def test_flux_f64_matches_oracle(dev, ora, method):
 def test_flux_f64_normals(dev, ora, normal):
     from asteriy_2d_amd import solver
     faces = facegen.faces(4096, seed=11)
-    for method in (G.HLL, G.HLLC, G.EXACT):
+    for method in (G.HLL, G.HLLC, G.EXACT, G.HLLC_A):
         out = solver.flux_batch(method, _t(faces, dev), *normal, 0.5).cpu().numpy()
         ref = ora.flux_batch(method, faces, *normal, 0.5)
         assert _close(out, ref, REL, np.maximum(np.abs(ref), 1e-300)) == 0
@@ -91,7 +91,7 @@ def _grids():
                                              G.BC_REFLECT))
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 @pytest.mark.parametrize("name,grid", list(_grids()), ids=[n for n, _ in _grids()])
 def test_step_f64_matches_oracle(dev, ora, name, grid, method):
     print(f"step_f64 {name} method={method} N={grid.N} M={grid.M}", flush=True)

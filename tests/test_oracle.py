@@ -152,7 +152,7 @@ def test_newton_and_devtwo_roots_satisfy_pressure_equation(ora):
 
 # ---- flux invariants (HLL kernel.cu:1399, HLLC Gas_Dinamic.cu:1604, exact) -------
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_consistency(ora, method):
     rng = np.random.default_rng(method)
     for _ in range(200):
@@ -166,14 +166,14 @@ def test_consistency(ora, method):
             assert np.abs(got - ref).max() <= 1e-13 * (np.abs(ref).max() + 1)
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_supersonic_upwinding(ora, method):
     # both sides supersonic to the right: the flux is the left physical flux
     L = ([1.0, 0.1], [5.0, 0.3])
     R = ([0.3, 2.0], [9.0, -0.2])
     Ps, Pu, _ = ora.flux(method, L[0], L[1], R[0], R[1], 1.0, 0.0, 1.0)
     rs, ru = phys_flux(1.0, 0.1, 5.0, 0.3, 1.0, 0.0, 5 / 3)
-    if method in (G.HLL, G.HLLC):  # early returns of the exact left flux
+    if method in (G.HLL, G.HLLC, G.HLLC_A):  # early returns of the exact left flux
         assert np.array_equal(Ps, rs) and np.array_equal(Pu, ru)
     else:
         assert np.allclose(Ps, rs, rtol=1e-14) and np.allclose(Pu, ru, rtol=1e-14)
@@ -184,7 +184,7 @@ def test_supersonic_upwinding(ora, method):
     assert np.allclose(Ps, rs, rtol=1e-14) and np.allclose(Pu, ru, rtol=1e-14)
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_mirror_antisymmetry(ora, method):
     """F(L, R, n) = -F(R, L, -n): exact for HLL (its sums only change sign),
     to round-off for HLLC (the SM numerator rounds in another order) and the
@@ -206,11 +206,49 @@ def test_stationary_contact(ora):
     """u = 0, p equal, rho different: HLLC and the exact solver resolve the
     contact (zero mass flux); HLL smears it."""
     L, R = ([1.0, 1.0], [0.0, 0.0]), ([0.1, 1.0], [0.0, 0.0])
-    for m in (G.HLLC, G.EXACT):
+    for m in (G.HLLC, G.EXACT, G.HLLC_A):
         Ps, Pu, _ = ora.flux(m, L[0], L[1], R[0], R[1], 1.0, 0.0, 1.0)
         assert Ps[0] == 0.0 and Ps[1] == 0.0 and Pu[0] == 1.0
     Ps, _, _ = ora.flux(G.HLL, L[0], L[1], R[0], R[1], 1.0, 0.0, 1.0)
     assert Ps[0] > 0.01
+
+
+def test_hllc_a_equals_hllc_at_equal_sound_speeds(ora):
+    """HLLC_Aleksashov (kernel.cu:767) and HLLC_2d_Korolkov (Gas_Dinamic.cu:1604)
+    differ only in their wave-speed estimates, min(uL - cL, uR - cR) vs
+    min(uL, uR) - max(cL, cR) (and the max likewise), which coincide when both
+    sides have the same sound speed: the fluxes then agree to round-off (the
+    SM numerator is summed in another order); with different sound speeds
+    they differ."""
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(500):
+        c2 = math.exp(rng.uniform(-2, 2))
+        rl, rr = math.exp(rng.uniform(-2, 2)), math.exp(rng.uniform(-2, 2))
+        ul, ur = rng.uniform(-3, 3, 2), rng.uniform(-3, 3, 2)
+        for n in ((1.0, 0.0), (0.0, -1.0)):
+            a = ora.flux(G.HLLC, [rl, rl * c2], ul, [rr, rr * c2], ur, *n, 0.3)
+            b = ora.flux(G.HLLC_A, [rl, rl * c2], ul, [rr, rr * c2], ur, *n, 0.3)
+            fa, fb = np.concatenate([a[0], a[1]]), np.concatenate([b[0], b[1]])
+            worst = max(worst, float(np.abs(fa - fb).max() / (np.abs(fa).max() + 1e-300)))
+            # (p/rho of the two sides can differ in the last bit, so can the speeds)
+            assert abs(a[2] - b[2]) <= 1e-14 * a[2]
+    assert worst < 1e-12
+    a = ora.flux(G.HLLC, [1.0, 1.0], [0.3, 0.0], [0.125, 0.1], [0.0, 0.0], 1.0, 0.0, 0.3)
+    b = ora.flux(G.HLLC_A, [1.0, 1.0], [0.3, 0.0], [0.125, 0.1], [0.0, 0.0], 1.0, 0.0, 0.3)
+    assert abs(a[0][0] - b[0][0]) > 1e-3 * abs(a[0][0])
+
+
+def test_hllc_a_dt_closed_form(ora):
+    """dt = krit*rad/max(|SL|, |SR|) with the per-side estimates
+    SL = min(uL - cL, uR - cR), SR = max(uL + cL, uR + cR) (kernel.cu:895-929)."""
+    for L, R in ((([1.0, 0.6], [0.2, 0.0]), ([0.5, 0.3], [-0.4, 0.1])),
+                 (([0.2, 2.0], [1.5, 0.0]), ([3.0, 0.1], [0.7, 0.2]))):
+        _, _, t = ora.flux(G.HLLC_A, L[0], L[1], R[0], R[1], 1.0, 0.0, 0.2, krit=0.3)
+        cl, cr = math.sqrt(5 / 3 * L[0][1] / L[0][0]), math.sqrt(5 / 3 * R[0][1] / R[0][0])
+        SL = min(L[1][0] - cl, R[1][0] - cr)
+        SR = max(L[1][0] + cl, R[1][0] + cr)
+        assert t == 0.3 * 0.2 / max(abs(SL), abs(SR))
 
 
 def test_hll_dt_closed_form(ora):
@@ -228,7 +266,7 @@ def _closed(N=24, M=16, geometry=G.PLANAR):
                               bc=(G.BC_REFLECT,) * 4)
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_rest_state_preserved_exactly(ora, method):
     for geometry in (G.PLANAR, G.AXISYMMETRIC):
         g = _closed(geometry=geometry)
@@ -240,7 +278,7 @@ def test_rest_state_preserved_exactly(ora, method):
         assert T < 1e7
 
 
-@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT])
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_free_stream_preserved(ora, method):
     g = G.Grid.from_extent(30, 12, 0.0, 1.0, 0.5, krit=0.3, geometry=G.PLANAR,
                            bc=(G.BC_OUTFLOW,) * 4)
