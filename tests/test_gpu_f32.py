@@ -136,6 +136,39 @@ def test_step_f32_discontinuous_vs_f64(dev, method):
     assert abs(a[2] - b[2]) <= 1e-5 * a[2]
 
 
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
+def test_step_f32_window_vs_f64(dev, method):
+    """A high-order window (kernel.cu:2701-2857: reconstructed states and
+    `method` only on faces with x in (ho_lo, ho_hi), first-order HLL on the
+    cell states elsewhere) with a frozen region straddling its edge: the FP32
+    step's window variant against the FP64 step, 5 steps of random data,
+    relative L1 1e-5."""
+    import torch
+    g = G.Grid.from_extent(190, 44, -1.0, 2.0, 1.2, krit=0.3, geometry=G.AXISYMMETRIC,
+                           bc=(G.BC_DIRICHLET, G.BC_OUTFLOW, G.BC_REFLECT, G.BC_OUTFLOW),
+                           bc_state=[[1.0, 0.8, 0.4, 0.0], [0.0] * 4, [0.0] * 4, [0.0] * 4],
+                           ho_lo=-0.2, ho_hi=1.3, frozen_x0=1.3, frozen_ax=0.25,
+                           frozen_ay=0.3)
+    s, u = problems.random_state(g, seed=9)
+    a = _run(g, s, u, 5, method, dev, torch.float64, T_do0=0.003)
+    b = _run(g, s, u, 5, method, dev, torch.float32, T_do0=0.003)
+    assert _l1(b[0][..., 0], a[0][..., 0]) < 1e-5
+    assert _l1(b[0][..., 1], a[0][..., 1]) < 1e-5
+    assert abs(a[2] - b[2]) <= 1e-5 * a[2]
+
+
+def test_reference_live_configuration_f32(dev):
+    """The configuration main() runs (a 640x8 slice), 20 FP32 steps against
+    the FP64 step: relative L1 of rho and p 1e-6 (smooth inflow data)."""
+    import torch
+    g = G.reference_live_grid(N=640, M=8)
+    s, u = problems.random_state(g, seed=5, smooth=True)
+    a = _run(g, s, u, 20, G.HLLC, dev, torch.float64, T_do0=1e-4)
+    b = _run(g, s, u, 20, G.HLLC, dev, torch.float32, T_do0=1e-4)
+    assert _l1(b[0][..., 0], a[0][..., 0]) < 1e-6
+    assert _l1(b[0][..., 1], a[0][..., 1]) < 1e-6
+
+
 def test_sod_f32(dev, ora):
     import torch
     p = problems.sod()
