@@ -230,6 +230,14 @@ AST_DEV bool frozen_possible(const ast_grid_t& g, int n0, int n1, int m0, int m1
 struct StripArgs {
   int n, m0, m1;
   bool cols_in, out_col, xvote, ho_x, ho_y;
+  // fast strips of the first / last band: the ghost lanes (gmask) apply
+  // their side's rule to the cell they load (col_off): vx -> med3(gsign * vx,
+  // glo, ghi) (mirror, clamped outflow), or the Dirichlet state gq (gdir);
+  // all wave-uniform
+  unsigned long long gmask;
+  bool gdir;
+  float gsign, glo, ghi;
+  float gq[4];
   float ax, ay;   // -T_do/dx, -T_do/dy
   float tdof, dxf, dyf, x, ymin;
   unsigned col_off;  // byte offset of the lane's column in a row (interior strips)
@@ -257,11 +265,18 @@ struct Row {
 // stores of the lanes that own no output go to an offset beyond the store
 // range and are dropped by the range check, so every row issues the same
 // memory operations and the load of row m + 3 never waits for a store.
-template <int METHOD, int GEOM, bool WINDOW, bool FAST, class ST>
+// Strip paths: kEdge -- load() with the ghost and clamping rules per cell;
+// kFast -- raw buffer loads of interior columns; kFastX -- the same for a
+// band that holds x-ghost columns, each lane loading the cell its ghost rule
+// names and applying the rule's per-lane transform.
+constexpr int kEdge = 0, kFast = 1, kFastX = 2;
+
+template <int METHOD, int GEOM, bool WINDOW, int PATH, class ST>
 AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, const ExactCtx& ex,
                        const float2* __restrict__ s, const float2* __restrict__ u,
                        float2* __restrict__ s2, float2* __restrict__ u2, const StripArgs& a,
                        float& UUx, float& UUy, ST& st) {
+  constexpr bool FAST = PATH != kEdge;
   const int n = a.n, m0 = a.m0, m1 = a.m1, N = g.N;
   const unsigned rb = a.row_bytes;
   // loads: FAST strips read rows m0-2 .. m1+1 through one resource per plane
@@ -272,6 +287,12 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     if constexpr (FAST) {
       const unsigned so = (unsigned)(m - (m0 - 2)) * rb;
       const float2 p = bload(rs, a.col_off, so), q = bload(ru, a.col_off, so);
+      if constexpr (PATH == kFastX) {  // kernel.cu:2549-2580, 2639-2654 (x sides)
+        const bool gl = (a.gmask >> (threadIdx.x & 63)) & 1ull;
+        const bool gd = gl & a.gdir;  // (no branch: the loads stay a row ahead)
+        const float vx = gl ? __builtin_amdgcn_fmed3f(a.gsign * q.x, a.glo, a.ghi) : q.x;
+        return Q{gd ? a.gq[0] : p.x, gd ? a.gq[1] : p.y, gd ? a.gq[2] : vx, gd ? a.gq[3] : q.y};
+      }
       return Q{p.x, p.y, q.x, q.y};
     } else {
       return load(g, s, u, n, m, a.cols_in);
@@ -316,6 +337,13 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
   // (memory operations retire in order: a store issued before the fetch of
   // row m+3 would make the wait for row m+2 wait for the store too).  The
   // first pending store goes to the drop offset.
+  // FAST axisymmetric strips: T_do/y of row m0 + k in lane k (the strip has
+  // at most 64 rows), read back per row as a scalar -- the per-row division
+  // by the wave-uniform radius costs one lane read instead of a conversion,
+  // an fma, a reciprocal and a product
+  float tyl = 0.f;
+  if constexpr (FAST && GEOM == AST_AXISYMMETRIC)
+    tyl = a.tdof * frcp(a.ymin + (m0 + (int)(threadIdx.x & 63)) * a.dyf);
   Q pend = w[0];
   unsigned pend_vo = kOffDrop;
 
@@ -374,7 +402,9 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     const float mx = q0.r * q0.vx, my = q0.r * q0.vy;
     if constexpr (GEOM == AST_AXISYMMETRIC) {
       // KorolkovSolver.cu:13-32: -(T_do/y) v {rho, rho u, rho v, E + p}
-      const float ty = a.tdof * frcp(y);
+      float ty;
+      if constexpr (FAST) ty = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tyl), m - m0));
+      else ty = a.tdof * frcp(y);
       const float tr = ty * my;
       dr_ -= tr;
       dmx -= tr * q0.vx;
@@ -448,6 +478,27 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
   a.ymin = (float)g.y_min;
   a.row_bytes = (unsigned)g.N * 8u;
   a.col_off = (unsigned)(a.n < 0 ? 0 : a.n) * 8u;
+  // the x-ghost rule of fetch_cell for the ghost lanes: only the first or the
+  // last band holds ghost columns, never both (more than one band)
+  const bool xlo_band = band * WOUT - 2 < 0, xhi_band = band * WOUT + 61 >= g.N;
+  const bool xfast = (xlo_band != xhi_band);  // wave-uniform
+  const int side = xlo_band ? AST_XLO : AST_XHI;
+  const int gbc = g.bc[side];
+  // lanes of columns -2, -1 (first band) or N, N+1 (last band)
+  const int l0 = xlo_band ? 0 : g.N - band * WOUT + 2;
+  a.gmask = xfast ? (3ull << l0) : 0ull;
+  a.gdir = gbc == AST_BC_DIRICHLET;
+  a.gsign = gbc == AST_BC_REFLECT ? -1.f : 1.f;
+  const float vc = (float)g.bc_vclamp[side];
+  a.glo = (gbc == AST_BC_OUTFLOW_CLAMPED && side == AST_XHI) ? vc : -3.40282347e38f;
+  a.ghi = (gbc == AST_BC_OUTFLOW_CLAMPED && side == AST_XLO) ? vc : 3.40282347e38f;
+  for (int k = 0; k < 4; k++) a.gq[k] = (float)g.bc_state[side][k];
+  if (xfast) {  // the cell each lane loads: ghost rule, else clamped into the grid
+    int src = a.n < 0 ? 0 : (a.n >= g.N ? g.N - 1 : a.n);
+    const bool ghost = (a.n < 0 && a.n >= -2) || (a.n >= g.N && a.n <= g.N + 1);
+    if (ghost && gbc == AST_BC_REFLECT) src = a.n < 0 ? -1 - a.n : 2 * g.N - 1 - a.n;
+    a.col_off = (unsigned)src * 8u;
+  }
   a.st_off = a.out_col ? (unsigned)a.n * 8u : kOffDrop;
   a.ho_x = high_order(g, g.x_min + a.n * g.dx - 0.5 * g.dx);  // west face window
   a.ho_y = high_order(g, g.x_min + a.n * g.dx);               // y faces: cell x
@@ -457,12 +508,16 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
               frozen_possible(g, band * WOUT - 3, band * WOUT + 62, a.m0 - 2, a.m0 + krows + 1);
   if (a.m0 < g.M) {
     // wave-uniform
-    const bool fast = a.cols_in && a.m0 - 2 >= 0 && a.m1 + 1 < g.M && !cw.frozen &&
+    const bool fast = (a.cols_in || xfast) && a.m0 - 2 >= 0 && a.m1 + 1 < g.M && !cw.frozen &&
+                      a.m1 - a.m0 <= 64 &&
                       ((a.m1 - a.m0) & 3) == 0;
     if (fast)
-      run_strip<METHOD, GEOM, WINDOW, true>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      if (a.cols_in)
+        run_strip<METHOD, GEOM, WINDOW, kFast>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      else
+        run_strip<METHOD, GEOM, WINDOW, kFastX>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
     else
-      run_strip<METHOD, GEOM, WINDOW, false>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      run_strip<METHOD, GEOM, WINDOW, kEdge>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
   }
   // only output lanes vote (halo lanes may hold garbage speeds)
   UUx = wave_max(a.xvote ? UUx : 0.f);
