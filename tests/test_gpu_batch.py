@@ -15,17 +15,18 @@ def _instances():
     return batch.sweep(count=5, seed=3, shapes=[(256, 128), (130, 70), (300, 64), (64, 200)])
 
 
-def test_batch_equals_single_grid(dev):
+@pytest.mark.parametrize("method", [G.HLLC, G.HLLC_A])
+def test_batch_equals_single_grid(dev, method):
     from asteriy_2d_amd.batch import Batch
     from asteriy_2d_amd.sim import Simulation
     inst = _instances()
-    with Batch([g for g, _, _, _ in inst], G.HLLC, [(s, u) for _, s, u, _ in inst]) as b:
+    with Batch([g for g, _, _, _ in inst], method, [(s, u) for _, s, u, _ in inst]) as b:
         b.run(23)
         got = [b.state(k) for k in range(len(inst))]
         times = [b.time(k) for k in range(len(inst))]
     for k, (g, s, u, _) in enumerate(inst):
         print(f"instance {k}: {g.N}x{g.M}", flush=True)
-        with Simulation(g, G.HLLC, "f32", s, u) as sim:
+        with Simulation(g, method, "f32", s, u) as sim:
             sim.run(23)
             rs, ru = sim.state()
             t = sim.time()

@@ -51,7 +51,10 @@ def test_flux_f32_vs_f64(dev, method):
     # component.  Measured on MI355X (median / p99 / p99.9): HLL 4e-9 / 5e-7 /
     # 2e-6, HLLC 3e-9 / 4.8e-7 / 2.7e-6 (the contact form of the star flux;
     # the form F_K + S_K (U*_K - U_K) had a 1e-4 / 5e-3 tail on the extreme
-    # pairs), exact 3e-9 / 1.6e-6 / 1.4e-5.  The tolerances below bound them.
+    # pairs), exact 3e-9 / 1.6e-6 / 1.4e-5; HLLC_Aleksashov, whose per-side
+    # wave speeds put S_K next to u_K on a cold side, stays inside the same
+    # bounds with its cancellation-free star terms (without them its p99.9
+    # was 6e-4).  The tolerances below bound them.
     F = facegen.faces(1 << 18, seed=21)
     F32 = _t(F, dev).float()
     st = solver.new_stats(dev)
@@ -359,6 +362,26 @@ def test_driver_matches_runtime(dev, tmp_path, precision):
     axis = np.loadtxt(tmp_path / "axis.txt")
     assert axis.shape == (p.grid.N, 7)
     assert np.allclose(axis[:, 2], s[0, :, 0], rtol=1e-5)
+
+
+def test_driver_hllc_aleksashov(dev, tmp_path):
+    """asteriy_run --method hllc_a (add_MK's method 1 flux) steps the Sod
+    tube like the runtime does (B31 state bit for bit)."""
+    import subprocess
+    from asteriy_2d_amd import build
+    from asteriy_2d_amd.sim import Simulation
+    out = tmp_path / "B31.bin"
+    r = subprocess.run([build.EXE, "--problem", "sod", "--steps", "30", "--print-every", "30",
+                        "--method", "hllc_a", "--out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.fromfile(out, dtype=np.float64).reshape(-1, 6)
+    p = problems.sod()
+    with Simulation(p.grid, G.HLLC_A, "f64", p.s, p.u) as sim:
+        sim.run(30)
+        s, u = sim.state()
+    assert np.array_equal(rec[:, 2], s[..., 0].ravel())
+    assert np.array_equal(rec[:, 4], u[..., 0].ravel())
 
 
 def test_driver_reference_live_configuration(dev):
