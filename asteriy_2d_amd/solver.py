@@ -1,8 +1,9 @@
 """Host-side mirror of the reference's operator interface for the step.
 
 The reference exposes (kernel.cu, Gas_Dinamic.cu):
-  * per-face operators  HLL / HLLC_2d_Korolkov / Godunov_Solver_Alexashov
-    (Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad) -> dt      kernel.cu:1399, 128;
+  * per-face operators  HLL / HLLC_2d_Korolkov / Godunov_Solver_Alexashov /
+    HLLC_Aleksashov
+    (Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad) -> dt      kernel.cu:1399, 128, 767;
                                                      Gas_Dinamic.cu:1814
   * the step kernel     Kernel_TVD(s, u, s2, u2, T, T_do, TT, method)
                                                      kernel.cu:2513
@@ -18,7 +19,7 @@ from __future__ import annotations
 import ctypes
 
 from . import native
-from .grid import EXACT, HLL as M_HLL, HLLC as M_HLLC
+from .grid import EXACT, HLL as M_HLL, HLLC as M_HLLC, HLLC_A as M_HLLC_A
 
 _STATS_WORDS = 10
 
@@ -118,6 +119,12 @@ def HLLC_2d_Korolkov(faces, n1, n2, rad, **kw):
 def Godunov_Solver_Alexashov(faces, n1, n2, rad, **kw):
     """kernel.cu:128 over a batch of faces."""
     return flux_batch(EXACT, faces, n1, n2, rad, **kw)
+
+
+def HLLC_Aleksashov(faces, n1, n2, rad, **kw):
+    """kernel.cu:767 (zero magnetic field, as add_MK calls it) over a batch of
+    faces."""
+    return flux_batch(M_HLLC_A, faces, n1, n2, rad, **kw)
 
 
 def conservation_sums(grid, s, u, stream=None):
