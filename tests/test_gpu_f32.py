@@ -160,6 +160,30 @@ def test_step_f32_window_vs_f64(dev, method):
     assert abs(a[2] - b[2]) <= 1e-5 * a[2]
 
 
+@pytest.mark.parametrize("bc", [G.BC_OUTFLOW, G.BC_REFLECT, G.BC_DIRICHLET, G.BC_OUTFLOW_CLAMPED])
+@pytest.mark.parametrize("N", [127, 178])
+def test_step_f32_x_ghost_rules_vs_f64(dev, bc, N):
+    """The first and last bands run the fast strip path with per-lane ghost
+    rules (mirror column, clamped column, Dirichlet state, clamped normal
+    velocity): each x-side rule on both sides, with the last band's ghost
+    lanes at two positions (N = 127: columns 120..126 in the last band;
+    N = 178: 58 columns), 3 steps of random data against the FP64 step,
+    relative L1 1e-5."""
+    import torch
+    g = G.Grid.from_extent(N, 40, -1.0, 2.0, 1.0, krit=0.3, geometry=G.AXISYMMETRIC,
+                           bc=(bc, bc, G.BC_REFLECT, G.BC_OUTFLOW),
+                           bc_state=[[0.9, 0.7, 0.3, 0.1], [0.6, 1.2, -0.4, 0.0], [0.0] * 4,
+                                     [0.0] * 4],
+                           bc_vclamp=(0.2, -0.1, 0.0, 0.0))
+    s, u = problems.random_state(g, seed=11)
+    a = _run(g, s, u, 3, G.HLLC, dev, torch.float64, T_do0=0.003)
+    b = _run(g, s, u, 3, G.HLLC, dev, torch.float32, T_do0=0.003)
+    assert _l1(b[0][..., 0], a[0][..., 0]) < 1e-5
+    assert _l1(b[0][..., 1], a[0][..., 1]) < 1e-5
+    assert _l1(b[1][..., 0], a[1][..., 0]) < 1e-4
+    assert abs(a[2] - b[2]) <= 1e-5 * a[2]
+
+
 def test_reference_live_configuration_f32(dev):
     """The configuration main() runs (a 640x8 slice), 20 FP32 steps against
     the FP64 step: relative L1 of rho and p 1e-6 (smooth inflow data)."""
