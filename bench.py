@@ -240,11 +240,13 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
-    ap.add_argument("--profile", choices=["f32", "f64", "exact32", "exact32l", "exact64"],
+    ap.add_argument("--profile", choices=["f32", "f64", "exact32", "exact32l", "exact64", "batch",
+                                          "flux"],
                     default=None,
                     help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
                          "step in FP32 / FP64 from the initial state, or the exact-solver step "
-                         "(FP32, FP32 with the learned guess, FP64)")
+                         "(FP32, FP32 with the learned guess, FP64); or --steps steps of the "
+                         "64-instance sweep (batch); or the Riemann-flux microbenchmark (flux)")
     ap.add_argument("--rows", type=int, default=0,
                     help="--profile: FP32 strip height (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -276,6 +278,20 @@ def main(argv=None):
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copied {K} x {src.numel() * 4} bytes")
+        return 0
+    if a.profile == "batch":
+        from asteriy_2d_amd import batch as B
+        inst = B.sweep(64, seed=5)
+        bt = B.Batch([x[0] for x in inst], HLLC, [(x[1], x[2]) for x in inst])
+        bt.autotune(reps=3)
+        bt.run(K)
+        bt.sync()
+        log(f"batch: {K} steps of {bt.cells} cells")
+        bt.close()
+        return 0
+    if a.profile == "flux":
+        from asteriy_2d_amd import learned
+        log(f"flux: {_flux_microbenchmark(learned.LearnedGuess.load())}")
         return 0
     if a.profile in ("exact32", "exact32l", "exact64"):
         f32 = a.profile != "exact64"
