@@ -39,6 +39,9 @@ SOURCES = [
     ("step_f64.hip", PARITY),
     ("flux_f32.hip", FAST),
     ("step_f32.hip", FAST),
+    ("step_f32_hll.hip", FAST),
+    ("step_f32_exact.hip", FAST),
+    ("step_f32_learned.hip", FAST),
     ("mlp.hip", FAST),
     ("batch.hip", FAST),
 ]

@@ -1,0 +1,132 @@
+// step_f32_kernel.hpp -- the fast step kernel (described in step_f32.hip)
+// and its launch over geometry, window and counter variants.  Each method's
+// kernels are instantiated in a source of their own (step_f32_<method>.hip),
+// so that the build compiles them in parallel; step_f32.hip holds the host
+// entry point.
+#pragma once
+
+#include <type_traits>
+
+#include "strip_f32.hpp"
+
+namespace ast {
+namespace f32 {
+
+// the arguments of one step launch, checked by the host entry point
+struct StepArgs {
+  ast_grid_t g;
+  Consts c;
+  ExactCtx ex;
+  bool window;
+  const float2* s;
+  const float2* u;
+  float2* s2;
+  float2* u2;
+  double* T;
+  const double* T_do;
+  KTime kt;
+  ast_stats_t* stats;
+  int krows;
+  hipStream_t stream;
+};
+
+// launches one method's kernel for a.g's geometry and a.window; defined
+// below, instantiated once per method in its own source
+template <int METHOD>
+int step_launch(const StepArgs& a);
+
+extern template int step_launch<AST_HLL>(const StepArgs&);
+extern template int step_launch<AST_HLLC>(const StepArgs&);
+extern template int step_launch<AST_HLLC_A>(const StepArgs&);
+extern template int step_launch<AST_EXACT>(const StepArgs&);
+extern template int step_launch<kExactLearned>(const StepArgs&);
+extern template int step_launch<kExactDetect>(const StepArgs&);
+
+#ifdef AST_STEP_F32_KERNELS
+
+// STATS: the variant that counts the exact solver's work into *stats
+template <int METHOD, int GEOM, bool WINDOW, bool STATS>
+__global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactCtx ex,
+                                                  const float2* __restrict__ s,
+                                                  const float2* __restrict__ u,
+                                                  float2* __restrict__ s2,
+                                                  float2* __restrict__ u2, double* T,
+                                                  const double* __restrict__ T_do, KTime kt,
+                                                  ast_stats_t* stats, int nbands, int krows,
+                                                  int nwg) {
+  __shared__ double red[WPB];
+  __shared__ uint32_t redi[WPB * 10];
+  // gated launch: this kernel's mode is not the selected one (grid-uniform)
+  if (ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
+  const double tdo = step_dt(T, T_do, kt);
+  constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
+  typename std::conditional<STATS, StatsF32,
+                            typename std::conditional<kDetects, StrongStats, NoStats>::type>::type st;
+  st.clear();
+  double tmin = INFINITY;
+  if constexpr (METHOD == kExactLearned) {
+    // several strip groups per workgroup when the grid is smaller than the
+    // work: the learned kernel runs on its co-resident capacity, so that it
+    // costs little in the steps in which it only reads the gate (the loop
+    // costs the other kernels registers, so they take one group each)
+    for (int item = blockIdx.x; item < nwg; item += gridDim.x) {  // workgroup-uniform
+      const int wg = xcd_tile(item, nwg);
+      tmin = fmin(tmin, strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo,
+                                                            wg % nbands, wg / nbands, krows, st));
+    }
+  } else {
+    const int wg = xcd_tile(blockIdx.x, nwg);
+    tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, wg % nbands,
+                                               wg / nbands, krows, st);
+  }
+  if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
+  if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
+  block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
+}
+
+template <int METHOD, int GEOM, bool WINDOW>
+int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
+                      const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
+                      const KTime& kt, ast_stats_t* stats, int krows, hipStream_t stream) {
+  const int nbands = (g.N + WOUT - 1) / WOUT;
+  const int nstrips = (g.M + krows - 1) / krows;
+  const int nsg = (nstrips + WPB - 1) / WPB;
+  const long long nwg = (long long)nbands * nsg;
+  if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
+  // the gated learned kernel: 512 workgroups (two waves per SIMD, what its
+  // registers allow to be resident at once) that loop over the strip groups
+  const unsigned grid =
+      (unsigned)(METHOD == kExactLearned && ex.gate != nullptr && nwg > 512 ? 512 : nwg);
+  if (is_exact(METHOD) && stats != nullptr) {
+    if constexpr (is_exact(METHOD))
+      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3(grid), dim3(NT), 0,
+                         stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
+                         (int)nwg);
+  } else {
+    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false>), dim3(grid), dim3(NT), 0,
+                       stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
+                       (int)nwg);
+  }
+  AST_LAUNCH_CHECK("ast_step_f32 launch");
+  return AST_OK;
+}
+
+template <int METHOD>
+int step_launch(const StepArgs& a) {
+  const bool axi = a.g.geometry == AST_AXISYMMETRIC;
+#define AST_LAUNCH(G, W)                                                                     \
+  launch_f32<METHOD, G, W>(a.g, a.c, a.ex, a.s, a.u, a.s2, a.u2, a.T, a.T_do, a.kt, a.stats, \
+                           a.krows, a.stream)
+  if constexpr (METHOD == kExactLearned || METHOD == kExactDetect) {  // without a window only
+    return axi ? AST_LAUNCH(AST_AXISYMMETRIC, false) : AST_LAUNCH(AST_PLANAR, false);
+  } else {
+    if (axi) return a.window ? AST_LAUNCH(AST_AXISYMMETRIC, true) : AST_LAUNCH(AST_AXISYMMETRIC, false);
+    return a.window ? AST_LAUNCH(AST_PLANAR, true) : AST_LAUNCH(AST_PLANAR, false);
+  }
+#undef AST_LAUNCH
+}
+
+#endif  // AST_STEP_F32_KERNELS
+
+}  // namespace f32
+}  // namespace ast
