@@ -238,6 +238,13 @@ struct StripArgs {
   bool gdir;
   float gsign, glo, ghi;
   float gq[4];
+  // fast edge strips (kFastE): the rules of the y-ghost rows, per side (0:
+  // y lo, 1: y hi), all wave-uniform: a ghost row loads the mirror row
+  // (yrefl) or the edge row, then vy -> med3(ysign * vy, ylo, yhi), or the
+  // Dirichlet state yq (ydir)
+  bool yrefl[2], ydir[2];
+  float ysign[2], ylo[2], yhi[2];
+  float yq[2][4];
   float ax, ay;   // -T_do/dx, -T_do/dy
   float tdof, dxf, dyf, x, ymin;
   unsigned col_off;  // byte offset of the lane's column in a row (interior strips)
@@ -251,12 +258,14 @@ struct Row {
   static constexpr int e0 = J & 1, e1 = (J + 1) & 1;
 };
 
-// One wave's strip.  FAST (wave-uniform): every load of the strip lies inside
-// the grid, no cell can be frozen and the strip has a multiple of 4 rows, so
-// rows are raw buffer loads at fixed per-lane offsets and no ghost, frozen
-// or row-count logic runs; otherwise load() applies the boundary rules, the
-// frozen tests run per row, and the rows of the last group of four that lie
-// past the strip vote for no dt (their stores fall outside the store range).
+// One wave's strip.  FAST (wave-uniform): the strip has a multiple of 4 rows
+// (at most 64) and its columns are interior or its band's ghost columns
+// follow a per-lane rule, so rows are raw buffer loads at fixed per-lane
+// offsets and no row-count logic runs (kFast, kFastX: no ghost row, no frozen
+// cell either; kFastE: ghost rows by row rules, frozen tests per row);
+// otherwise load() applies the boundary rules, the frozen tests run per row,
+// and the rows of the last group of four that lie past the strip vote for no
+// dt (their stores fall outside the store range).
 //
 // Rows are processed four per trip with the window held in w[4] (row r in
 // slot (r - m0) & 3): the slide of the window is a change of names, not a
@@ -268,8 +277,11 @@ struct Row {
 // Strip paths: kEdge -- load() with the ghost and clamping rules per cell;
 // kFast -- raw buffer loads of interior columns; kFastX -- the same for a
 // band that holds x-ghost columns, each lane loading the cell its ghost rule
-// names and applying the rule's per-lane transform.
-constexpr int kEdge = 0, kFast = 1, kFastX = 2;
+// names and applying the rule's per-lane transform; kFastE -- a strip that
+// holds y-ghost rows or may meet frozen cells: the x-ghost lanes of kFastX,
+// the row a ghost row loads chosen in scalar code and the y rule applied to
+// the whole row, the frozen tests of kEdge.
+constexpr int kEdge = 0, kFast = 1, kFastX = 2, kFastE = 3;
 
 template <int METHOD, int GEOM, bool WINDOW, int PATH, class ST>
 AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, const ExactCtx& ex,
@@ -277,21 +289,44 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
                        float2* __restrict__ s2, float2* __restrict__ u2, const StripArgs& a,
                        float& UUx, float& UUy, ST& st) {
   constexpr bool FAST = PATH != kEdge;
-  const int n = a.n, m0 = a.m0, m1 = a.m1, N = g.N;
+  const int n = a.n, m0 = a.m0, m1 = a.m1, N = g.N, M = g.M;
   const unsigned rb = a.row_bytes;
   // loads: FAST strips read rows m0-2 .. m1+1 through one resource per plane
-  const size_t b0 = FAST ? (size_t)(m0 - 2) * N : 0;
-  const long long lbytes = FAST ? (long long)(m1 - m0 + 4) * rb : 0;
+  // (kFastE: the rows of the grid among them, which hold every row a ghost
+  // row names; a strip of kFastE has at least 4 rows, so M >= 4)
+  const int rlo = PATH == kFastE ? max(m0 - 2, 0) : m0 - 2;
+  const int rhi = PATH == kFastE ? min(m1 + 1, M - 1) : m1 + 1;
+  const size_t b0 = FAST ? (size_t)rlo * N : 0;
+  const long long lbytes = FAST ? (long long)(rhi - rlo + 1) * rb : 0;
   const __amdgpu_buffer_rsrc_t rs = make_rsrc(s + b0, lbytes), ru = make_rsrc(u + b0, lbytes);
   auto LD = [&](int m) -> Q {
     if constexpr (FAST) {
-      const unsigned so = (unsigned)(m - (m0 - 2)) * rb;
+      // kernel.cu:2564-2569, 2609-2614, 2661-2698 (y sides): the row a ghost
+      // row loads (wave-uniform)
+      const bool ylo_row = PATH == kFastE && m < 0, yhi_row = PATH == kFastE && m >= M;
+      int ms = m;
+      if (ylo_row) ms = a.yrefl[0] ? -1 - m : 0;
+      if (yhi_row) ms = a.yrefl[1] ? 2 * M - 1 - m : M - 1;
+      const unsigned so = (unsigned)(ms - rlo) * rb;
       const float2 p = bload(rs, a.col_off, so), q = bload(ru, a.col_off, so);
-      if constexpr (PATH == kFastX) {  // kernel.cu:2549-2580, 2639-2654 (x sides)
+      if constexpr (PATH == kFastX || PATH == kFastE) {  // kernel.cu:2549-2580, 2639-2654 (x sides)
         const bool gl = (a.gmask >> (threadIdx.x & 63)) & 1ull;
         const bool gd = gl & a.gdir;  // (no branch: the loads stay a row ahead)
         const float vx = gl ? __builtin_amdgcn_fmed3f(a.gsign * q.x, a.glo, a.ghi) : q.x;
-        return Q{gd ? a.gq[0] : p.x, gd ? a.gq[1] : p.y, gd ? a.gq[2] : vx, gd ? a.gq[3] : q.y};
+        Q r{gd ? a.gq[0] : p.x, gd ? a.gq[1] : p.y, gd ? a.gq[2] : vx, gd ? a.gq[3] : q.y};
+        if constexpr (PATH == kFastE) {  // the y rule, the same for the whole row
+          const bool gy = ylo_row || yhi_row;
+          const float sg = yhi_row ? a.ysign[1] : a.ysign[0];
+          const float lo = yhi_row ? a.ylo[1] : a.ylo[0];
+          const float hi = yhi_row ? a.yhi[1] : a.yhi[0];
+          const bool yd = gy && (yhi_row ? a.ydir[1] : a.ydir[0]);
+          const float vy = gy ? __builtin_amdgcn_fmed3f(sg * r.vy, lo, hi) : r.vy;
+          r = Q{yd ? (yhi_row ? a.yq[1][0] : a.yq[0][0]) : r.r,
+                yd ? (yhi_row ? a.yq[1][1] : a.yq[0][1]) : r.p,
+                yd ? (yhi_row ? a.yq[1][2] : a.yq[0][2]) : r.vx,
+                yd ? (yhi_row ? a.yq[1][3] : a.yq[0][3]) : vy};
+        }
+        return r;
       }
       return Q{p.x, p.y, q.x, q.y};
     } else {
@@ -306,7 +341,7 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
   const long long sbytes = (long long)(m1 - m0) * rb;
   const __amdgpu_buffer_rsrc_t ws = make_rsrc(s2 + sb, sbytes), wu = make_rsrc(u2 + sb, sbytes);
 
-  const bool frz = !FAST && cw.frozen;  // wave-uniform
+  const bool frz = (PATH == kEdge || PATH == kFastE) && cw.frozen;  // wave-uniform
 
   // prologue: rows m0-2 .. m0+1, the south face of row m0
   Q w[4], nv[2], dd[2];
@@ -508,16 +543,31 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
               frozen_possible(g, band * WOUT - 3, band * WOUT + 62, a.m0 - 2, a.m0 + krows + 1);
   if (a.m0 < g.M) {
     // wave-uniform
-    const bool fast = (a.cols_in || xfast) && a.m0 - 2 >= 0 && a.m1 + 1 < g.M && !cw.frozen &&
-                      a.m1 - a.m0 <= 64 &&
-                      ((a.m1 - a.m0) & 3) == 0;
-    if (fast)
+    const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 <= 64 && ((a.m1 - a.m0) & 3) == 0;
+    const bool rows_in = a.m0 - 2 >= 0 && a.m1 + 1 < g.M;
+    if (fast_rows && rows_in && !cw.frozen) {
       if (a.cols_in)
         run_strip<METHOD, GEOM, WINDOW, kFast>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
       else
         run_strip<METHOD, GEOM, WINDOW, kFastX>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
-    else
+    } else if (fast_rows) {
+      // the y-ghost rules of fetch_cell as row rules (only the first or last
+      // strips of a band hold ghost rows)
+      for (int k = 0; k < 2; k++) {
+        const int ys = k ? AST_YHI : AST_YLO;
+        const int ybc = g.bc[ys];
+        const float yc = (float)g.bc_vclamp[ys];
+        a.yrefl[k] = ybc == AST_BC_REFLECT;
+        a.ydir[k] = ybc == AST_BC_DIRICHLET;
+        a.ysign[k] = ybc == AST_BC_REFLECT ? -1.f : 1.f;
+        a.ylo[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 1) ? yc : -3.40282347e38f;
+        a.yhi[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 0) ? yc : 3.40282347e38f;
+        for (int j = 0; j < 4; j++) a.yq[k][j] = (float)g.bc_state[ys][j];
+      }
+      run_strip<METHOD, GEOM, WINDOW, kFastE>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+    } else {
       run_strip<METHOD, GEOM, WINDOW, kEdge>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+    }
   }
   // only output lanes vote (halo lanes may hold garbage speeds)
   UUx = wave_max(a.xvote ? UUx : 0.f);

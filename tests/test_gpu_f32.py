@@ -184,6 +184,43 @@ def test_step_f32_x_ghost_rules_vs_f64(dev, bc, N):
     assert abs(a[2] - b[2]) <= 1e-5 * a[2]
 
 
+@pytest.mark.parametrize("ybc", [(G.BC_REFLECT, G.BC_OUTFLOW),
+                                 (G.BC_OUTFLOW_CLAMPED, G.BC_OUTFLOW_CLAMPED),
+                                 (G.BC_DIRICHLET, G.BC_REFLECT),
+                                 (G.BC_OUTFLOW, G.BC_DIRICHLET)])
+@pytest.mark.parametrize("method,geometry", [(G.HLLC, G.AXISYMMETRIC), (G.EXACT, G.AXISYMMETRIC),
+                                             (G.HLL, G.PLANAR)])
+def test_step_f32_edge_strips_fast_equals_boundary_path(dev, ybc, method, geometry):
+    """Strips that hold y-ghost rows or may meet frozen cells take the fast
+    edge path (raw buffer loads, the y rule applied to whole rows, frozen
+    tests per row) at 8-row strips; at 6-row strips (not a multiple of 4)
+    every strip takes the per-cell boundary path.  Both paths do the same
+    arithmetic, so 3 steps agree bit for bit in state and time: each y-side
+    rule, Dirichlet and reflecting x sides (corner strips), and a frozen
+    region on the axis that reaches the second strip of its band."""
+    import torch
+    from asteriy_2d_amd import native
+    g = G.Grid.from_extent(150, 48, -1.0, 2.0, 1.0, krit=0.3, geometry=geometry,
+                           bc=(G.BC_DIRICHLET, G.BC_REFLECT, ybc[0], ybc[1]),
+                           bc_state=[[0.9, 0.7, 0.3, 0.1], [0.0] * 4, [0.8, 1.1, 0.2, 0.3],
+                                     [0.5, 0.6, -0.2, -0.3]],
+                           bc_vclamp=(0.0, 0.0, 0.15, -0.05),
+                           frozen_x0=0.4, frozen_ax=0.2, frozen_ay=0.15)
+    s, u = problems.random_state(g, seed=17)
+    L = native.load()
+    old = L.ast_get_tile_rows()
+    try:
+        L.ast_set_tile_rows(8)
+        a = _run(g, s, u, 3, method, dev, torch.float32, T_do0=0.003)
+        L.ast_set_tile_rows(6)
+        b = _run(g, s, u, 3, method, dev, torch.float32, T_do0=0.003)
+    finally:
+        L.ast_set_tile_rows(old)
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert a[2] == b[2]
+
+
 def test_reference_live_configuration_f32(dev):
     """The configuration main() runs (a 640x8 slice), 20 FP32 steps against
     the FP64 step: relative L1 of rho and p 1e-6 (smooth inflow data)."""
