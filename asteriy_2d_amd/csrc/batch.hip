@@ -36,9 +36,12 @@ struct BatchInst {
 template <int METHOD, int GEOM>
 AST_DEV void batch_item(const BatchInst& I, int local, int from, double* red) {
   NoStats st;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int band, m0, m1;
+  strip_rows(I.g.M, I.nbands, I.krows, local, wave, band, m0, m1);
   const double tmin = strip_of_wave<METHOD, GEOM, false>(
       I.g, I.c, I.ex, I.plane[2 * from], I.plane[2 * from + 1], I.plane[2 * (1 - from)],
-      I.plane[2 * (1 - from) + 1], I.scal[1], local % I.nbands, local / I.nbands, I.krows, st);
+      I.plane[2 * (1 - from) + 1], I.scal[1], band, m0, m1, st);
   block_min_uniform<NT>(tmin, red, I.scal, 0);
 }
 

@@ -276,9 +276,11 @@ extern "C" int ast_sim_set_mlp(ast_sim_t* S, const ast_mlp_t* mlp) {
   return AST_OK;
 }
 
-// Rows per wave strip of the FP32 step (its tile shape; see the autotuner).
+// Strip tiling of the FP32 step (its tile shape; see the autotuner): rows
+// per wave strip, or -S for S strips of (nearly) equal height per band.
 extern "C" int ast_sim_set_tile_rows(ast_sim_t* S, int krows) {
-  if (!S || krows < 1 || krows > 1024) return ast_fail(AST_ERR_ARG, "ast_sim_set_tile_rows");
+  if (!S || krows == 0 || krows > 1024 || (krows < 0 && -(long long)krows > S->g.M))
+    return ast_fail(AST_ERR_ARG, "ast_sim_set_tile_rows: %d", krows);
   CK(hipStreamSynchronize(S->stream));
   S->krows = krows;
   drop_graph(S);
@@ -288,7 +290,8 @@ extern "C" int ast_sim_set_tile_rows(ast_sim_t* S, int krows) {
 extern "C" void* ast_sim_stream(ast_sim_t* S) { return S ? (void*)S->stream : nullptr; }
 
 // Tile autotuner of the FP32 step: times `reps` launches of the step kernel
-// for each candidate strip height (rows per wave) on the simulation's own
+// for each candidate strip tiling (rows per wave strip, or -S: S strips of
+// (nearly) equal height per band) on the simulation's own
 // state and stream, keeps the fastest, and restores the time scalars (the
 // repeated launches read the current planes and only write the spare ones).
 // best_rows / best_us (may be NULL) report the choice.  The FP64 step has a
@@ -296,10 +299,21 @@ extern "C" void* ast_sim_stream(ast_sim_t* S) { return S ? (void*)S->stream : nu
 extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, int reps,
                                 int* best_rows, float* best_us) {
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_autotune: null sim");
-  static const int kDefault[] = {4, 8, 12, 16};  // multiples of 4 take the fast strip path
+  // default: strips of 4-16 rows, and the tilings that give every SIMD 4 or
+  // 5 waves of (nearly) equal height, S strips per band with S * bands
+  // waves (a wave strip has 60 output columns; a CU has 4 SIMDs)
+  int dflt[6] = {4, 8, 12, 16};
   if (!candidates || ncand <= 0) {
-    candidates = kDefault;
-    ncand = (int)(sizeof kDefault / sizeof kDefault[0]);
+    int nd = 4, dev = 0, cus = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int nbands = (S->g.N + 59) / 60;
+    for (int waves = 4; waves <= 5; waves++) {
+      const int ns = waves * 4 * cus / nbands;
+      if (ns >= 1 && S->g.M / ns >= 4) dflt[nd++] = -ns;
+    }
+    candidates = dflt;
+    ncand = nd;
   }
   if (reps < 1) reps = 10;
   double saved[4];
@@ -320,7 +334,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   const int n = S->precision == 1 ? ncand : 1;
   for (int c = 0; c < n; c++) {
     const int kr = S->precision == 1 ? candidates[c] : 0;
-    if (S->precision == 1 && (kr < 1 || kr > 1024)) continue;
+    if (S->precision == 1 && (kr == 0 || kr > 1024 || (kr < 0 && -kr > S->g.M))) continue;
     for (int r = -2; r < reps; r++) {  // two untimed launches first
       if (r == 0) CK(hipEventRecord(e0, S->stream));
       int rc = S->precision == 1

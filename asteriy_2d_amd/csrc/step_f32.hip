@@ -38,8 +38,11 @@ using namespace ast::f32;
 
 static int g_krows = 8;
 
+// krows > 0: rows per strip; krows = -S: S strips of (nearly) equal height
+// per band (strip_rows)
 extern "C" int ast_set_tile_rows(int krows) {
-  if (krows < 1 || krows > 1024) return ast_fail(AST_ERR_ARG, "ast_set_tile_rows: %d", krows);
+  if (krows == 0 || krows > 1024 || krows < -(1 << 20))
+    return ast_fail(AST_ERR_ARG, "ast_set_tile_rows: %d", krows);
   g_krows = krows;
   return AST_OK;
 }
@@ -63,16 +66,18 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
     return ast_fail(AST_ERR_ARG, "ast_step_f32: null state or time pointer");
   if (s == s2 || u == u2 || s == u2 || u == s2)
     return ast_fail(AST_ERR_ARG, "ast_step_f32: output planes must not alias the input planes");
-  if (krows < 1) return ast_fail(AST_ERR_ARG, "ast_step_f32: krows %d", krows);
+  if (krows == 0 || krows > 1024 || (krows < 0 && -(long long)krows > g->M))
+    return ast_fail(AST_ERR_ARG, "ast_step_f32: strip tiling %d for %d rows", krows, g->M);
+  const int height = krows > 0 ? krows : (g->M - krows - 1) / -krows;  // tallest strip
   KTime kt = {nullptr, nullptr, nullptr, 0};
   if (tm != nullptr) {
     if (tm->nslots < 1 || tm->nslots > kMaxSlots || tm->cur == nullptr)
       return ast_fail(AST_ERR_ARG, "ast_step_f32: %d dt slots", tm->nslots);
     kt = KTime{tm->prev, tm->cur, tm->rst, tm->nslots};
   }
-  if (!strip_fits(*g, krows))
+  if (!strip_fits(*g, height))
     return ast_fail(AST_ERR_SIZE, "ast_step_f32: %d columns x %d rows per strip exceed one buffer range",
-                    g->N, krows + 4);
+                    g->N, height + 4);
   rc = check_positive_bc(*g);
   if (rc != AST_OK) return rc;
   const Consts c = make_consts(*g);

@@ -64,20 +64,22 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                             typename std::conditional<kDetects, StrongStats, NoStats>::type>::type st;
   st.clear();
   double tmin = INFINITY;
+  // wave index made explicitly uniform: rows and the loop stay scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int band, m0, m1;
   if constexpr (METHOD == kExactLearned) {
     // several strip groups per workgroup when the grid is smaller than the
     // work: the learned kernel runs on its co-resident capacity, so that it
     // costs little in the steps in which it only reads the gate (the loop
     // costs the other kernels registers, so they take one group each)
     for (int item = blockIdx.x; item < nwg; item += gridDim.x) {  // workgroup-uniform
-      const int wg = xcd_tile(item, nwg);
-      tmin = fmin(tmin, strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo,
-                                                            wg % nbands, wg / nbands, krows, st));
+      strip_rows(g.M, nbands, krows, xcd_tile(item, nwg), wave, band, m0, m1);
+      tmin = fmin(tmin, strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, band,
+                                                            m0, m1, st));
     }
   } else {
-    const int wg = xcd_tile(blockIdx.x, nwg);
-    tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, wg % nbands,
-                                               wg / nbands, krows, st);
+    strip_rows(g.M, nbands, krows, xcd_tile(blockIdx.x, nwg), wave, band, m0, m1);
+    tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st);
   }
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
@@ -88,10 +90,15 @@ template <int METHOD, int GEOM, bool WINDOW>
 int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
                       const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
                       const KTime& kt, ast_stats_t* stats, int krows, hipStream_t stream) {
+  // krows > 0: strips of krows rows; krows = -S: S strips per band (strip_rows)
   const int nbands = (g.N + WOUT - 1) / WOUT;
-  const int nstrips = (g.M + krows - 1) / krows;
-  const int nsg = (nstrips + WPB - 1) / WPB;
-  const long long nwg = (long long)nbands * nsg;
+  long long nwg;
+  if (krows > 0) {
+    const int nstrips = (g.M + krows - 1) / krows;
+    nwg = (long long)nbands * ((nstrips + WPB - 1) / WPB);
+  } else {
+    nwg = ((long long)nbands * -(long long)krows + WPB - 1) / WPB;
+  }
   if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
   // the gated learned kernel: 512 workgroups (two waves per SIMD, what its
   // registers allow to be resident at once) that loop over the strip groups

@@ -258,11 +258,12 @@ struct Row {
   static constexpr int e0 = J & 1, e1 = (J + 1) & 1;
 };
 
-// One wave's strip.  FAST (wave-uniform): the strip has a multiple of 4 rows
-// (at most 64) and its columns are interior or its band's ghost columns
-// follow a per-lane rule, so rows are raw buffer loads at fixed per-lane
-// offsets and no row-count logic runs (kFast, kFastX: no ghost row, no frozen
-// cell either; kFastE: ghost rows by row rules, frozen tests per row);
+// One wave's strip.  FAST (wave-uniform): the strip has 4 to 64 rows and its
+// columns are interior or its band's ghost columns follow a per-lane rule,
+// so rows are raw buffer loads at fixed per-lane offsets (kFast, kFastX: no
+// ghost row, no frozen cell either, rows in groups of four and a tail of 1-3
+// rows; kFastE: ghost rows by row rules, frozen tests per row, a multiple of
+// 4 rows);
 // otherwise load() applies the boundary rules, the frozen tests run per row,
 // and the rows of the last group of four that lie past the strip vote for no
 // dt (their stores fall outside the store range).
@@ -470,11 +471,23 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     pend_vo = a.st_off + (unsigned)(m - m0) * rb;
   };
 
-  for (int m = m0; m < m1; m += 4) {
+  // kFast, kFastX: whole groups of four, then the 1-3 rows left
+  // (wave-uniform); kFastE strips are whole groups (a tail there costs the
+  // kernel registers); the boundary variant runs its last group whole (rows
+  // past m1 drop out)
+  constexpr bool TAIL = PATH == kFast || PATH == kFastX;
+  const int mq = TAIL ? m0 + ((m1 - m0) & ~3) : m1;
+  for (int m = m0; m < mq; m += 4) {
     row(Row<0>{}, m);
     row(Row<1>{}, m + 1);
     row(Row<2>{}, m + 2);
     row(Row<3>{}, m + 3);
+  }
+  if constexpr (TAIL) {
+    const int rest = m1 - mq;
+    if (rest > 0) row(Row<0>{}, mq);
+    if (rest > 1) row(Row<1>{}, mq + 1);
+    if (rest > 2) row(Row<2>{}, mq + 2);
   }
   bstore(pend.r, pend.p, ws, pend_vo, 0);
   bstore(pend.vx, pend.vy, wu, pend_vo, 0);
@@ -486,20 +499,54 @@ AST_DEV float wave_max(float v) {
   return v;
 }
 
-// The strip of one wave given its workgroup's (band, strip group): fills
+// The band and rows [m0, m1) of wave `wave` of workgroup `wg` under a strip
+// tiling (wave-uniform).  tile > 0: strips of `tile` rows, WPB of them
+// stacked per workgroup, band = wg mod nbands.  tile = -S: every band cut
+// into S strips of about M/S rows, the waves of all workgroups numbered band
+// by band (a workgroup may span two bands); this lets the launch hold
+// exactly as many waves as the SIMDs take at once.  The first and last
+// strip of a band (they hold the ghost rows) get M/S rounded to a multiple
+// of 4, the others floor or ceil of what is left over them.  Waves past the
+// last band get no rows (m0 = m1 = M).
+AST_DEV void strip_rows(int M, int nbands, int tile, int wg, int wave, int& band, int& m0,
+                        int& m1) {
+  if (tile > 0) {
+    band = wg % nbands;
+    m0 = ((wg / nbands) * WPB + wave) * tile;
+    m1 = min(m0 + tile, M);
+  } else {
+    const int S = -tile, gw = wg * WPB + wave;
+    band = gw / S;
+    const int k = gw - band * S;
+    const int e = max(4, (M / S + 2) / 4 * 4), mid = M - 2 * e;
+    if (S >= 3 && mid >= S - 2) {
+      const int h = mid / (S - 2), r = mid - h * (S - 2), j = k - 1;
+      m0 = k == 0 ? 0 : (k == S - 1 ? M - e : e + j * h + min(j, r));
+      m1 = k == 0 ? e : (k == S - 1 ? M : m0 + h + (j < r ? 1 : 0));
+    } else {
+      const int h = M / S, r = M - h * S;
+      m0 = k * h + min(k, r);
+      m1 = m0 + h + (k < r ? 1 : 0);
+    }
+    if (band >= nbands) {
+      band = nbands - 1;
+      m0 = m1 = M;
+    }
+  }
+}
+
+// The strip of one wave given its band and rows (strip_rows): fills
 // StripArgs and runs the fast or the boundary variant.  Returns the wave's
 // dt candidate (wave-uniform).
 template <int METHOD, int GEOM, bool WINDOW, class ST>
 AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCtx& ex,
                              const float2* __restrict__ s, const float2* __restrict__ u,
                              float2* __restrict__ s2, float2* __restrict__ u2, double tdo,
-                             int band, int sgrp, int krows, ST& st) {
-  // wave index made explicitly uniform: rows and the loop stay scalar
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+                             int band, int m0, int m1, ST& st) {
   const int lane = threadIdx.x & 63;
   StripArgs a;
-  a.m0 = (sgrp * WPB + wave) * krows;
-  a.m1 = min(a.m0 + krows, g.M);
+  a.m0 = m0;
+  a.m1 = m1;
   a.n = band * WOUT + lane - 2;
   a.cols_in = (band * WOUT - 2 >= 0) && (band * WOUT + 61 < g.N);  // wave-uniform
   a.out_col = (lane >= 2) & (lane < 2 + WOUT) & (a.n < g.N);
@@ -540,17 +587,17 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
   float UUx = 0.f, UUy = 0.f;
   Consts cw = c;  // frozen tests only where the wave can meet the region
   cw.frozen = c.frozen &&
-              frozen_possible(g, band * WOUT - 3, band * WOUT + 62, a.m0 - 2, a.m0 + krows + 1);
+              frozen_possible(g, band * WOUT - 3, band * WOUT + 62, a.m0 - 2, a.m1 + 1);
   if (a.m0 < g.M) {
     // wave-uniform
-    const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 <= 64 && ((a.m1 - a.m0) & 3) == 0;
+    const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 >= 4 && a.m1 - a.m0 <= 64;
     const bool rows_in = a.m0 - 2 >= 0 && a.m1 + 1 < g.M;
     if (fast_rows && rows_in && !cw.frozen) {
       if (a.cols_in)
         run_strip<METHOD, GEOM, WINDOW, kFast>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
       else
         run_strip<METHOD, GEOM, WINDOW, kFastX>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
-    } else if (fast_rows) {
+    } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0) {
       // the y-ghost rules of fetch_cell as row rules (only the first or last
       // strips of a band hold ghost rows)
       for (int k = 0; k < 2; k++) {

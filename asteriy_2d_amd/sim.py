@@ -129,8 +129,9 @@ class Simulation:
         native.call("ast_sim_set_tile_rows", self.h, int(krows))
 
     def autotune(self, candidates=None, reps: int = 10):
-        """Picks the fastest FP32 strip height for this grid (csrc/sim.cpp);
-        returns (rows, microseconds per step)."""
+        """Picks the fastest FP32 strip tiling for this grid (csrc/sim.cpp);
+        returns (tiling, microseconds per step): rows per wave strip, or -S
+        for S strips of (nearly) equal height per 60-column band."""
         arr, n = None, 0
         if candidates:
             n = len(candidates)

@@ -248,7 +248,8 @@ def main(argv=None):
                          "(FP32, FP32 with the learned guess, FP64); or --steps steps of the "
                          "64-instance sweep (batch); or the Riemann-flux microbenchmark (flux)")
     ap.add_argument("--rows", type=int, default=0,
-                    help="--profile: FP32 strip height (default: the autotuner's choice)")
+                    help="--profile: FP32 strip tiling, rows per strip or -S strips per band "
+                         "(default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer --steps times (counter calibration)")
     a = ap.parse_args(argv)
@@ -470,7 +471,8 @@ def main(argv=None):
                    "grid": [N_GRID, M_GRID], "method": "hllc", "cfl": 0.3, "gamma": 5 / 3},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": _traffic(f"step_f32_hllc_2048x1024_r{rows}")},
+                     "traffic": _traffic("step_f32_hllc_2048x1024_"
+                                         + (f"r{rows}" if rows > 0 else f"s{-rows}"))},
         "cpu_baseline": cpu,
         **extra,
     }

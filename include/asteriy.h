@@ -278,12 +278,14 @@ int ast_sim_collect_stats(ast_sim_t* sim, int on);
  * a device word, and only one of them runs -- the learned one in the steps
  * that follow a step with strong two-shock faces. */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
-/* Strip height (rows per wave) of the FP32 step; multiples of 4 take the
- * fast strip path. */
+/* Strip tiling of the FP32 step: rows > 0 rows per wave strip, rows = -S
+ * S strips of (nearly) equal height per band of 60 columns; strips of 4 to
+ * 64 rows inside the grid take the fast strip path. */
 int ast_sim_set_tile_rows(ast_sim_t* sim, int rows);
-/* Times the FP32 step for each candidate strip height (NULL = a default
- * list), keeps the fastest; reports it (may be NULL).  State and time are
- * left unchanged. */
+/* Times the FP32 step for each candidate tiling (NULL = a default list:
+ * 4, 8, 12, 16 rows and the two tilings -S that give every SIMD 4 or 5
+ * equal waves), keeps the fastest; reports it (may be NULL).  State and
+ * time are left unchanged. */
 int ast_sim_autotune(ast_sim_t* sim, const int* candidates, int ncand, int reps,
                      int* best_rows, float* best_us);
 /* The simulation's stream (hipStream_t) for event timing. */
@@ -329,7 +331,7 @@ void* ast_batch_stream(ast_batch_t* b);
 long long ast_batch_cells(const ast_batch_t* b);
 
 /* ---- tile configuration of the stand-alone FP32 step entry point --------
- * Rows per wave strip (default 8; multiples of 4 take the fast path). */
+ * Strip tiling as for ast_sim_set_tile_rows (default 8 rows per strip). */
 int ast_set_tile_rows(int rows);
 int ast_get_tile_rows(void);
 /* Doubles of device memory ast_conservation_sums needs at `out` (the two

@@ -190,14 +190,17 @@ def test_step_f32_x_ghost_rules_vs_f64(dev, bc, N):
                                  (G.BC_OUTFLOW, G.BC_DIRICHLET)])
 @pytest.mark.parametrize("method,geometry", [(G.HLLC, G.AXISYMMETRIC), (G.EXACT, G.AXISYMMETRIC),
                                              (G.HLL, G.PLANAR)])
-def test_step_f32_edge_strips_fast_equals_boundary_path(dev, ybc, method, geometry):
+@pytest.mark.parametrize("tile", [8, -5, -7, -9])
+def test_step_f32_edge_strips_fast_equals_boundary_path(dev, ybc, method, geometry, tile):
     """Strips that hold y-ghost rows or may meet frozen cells take the fast
     edge path (raw buffer loads, the y rule applied to whole rows, frozen
-    tests per row) at 8-row strips; at 6-row strips (not a multiple of 4)
-    every strip takes the per-cell boundary path.  Both paths do the same
-    arithmetic, so 3 steps agree bit for bit in state and time: each y-side
-    rule, Dirichlet and reflecting x sides (corner strips), and a frozen
-    region on the axis that reaches the second strip of its band."""
+    tests per row); with tile = -S the 48 rows of a band are cut into S
+    strips (ends of 8 or 4 rows, the others 5-11 rows: interior strips end
+    with a tail of 1-3 rows).  At 6-row strips every strip takes the
+    per-cell boundary path.  All paths do the same arithmetic, so 3 steps
+    agree bit for bit in state and time: each y-side rule, Dirichlet and
+    reflecting x sides (corner strips), and a frozen region on the axis that
+    reaches the second strip of its band."""
     import torch
     from asteriy_2d_amd import native
     g = G.Grid.from_extent(150, 48, -1.0, 2.0, 1.0, krit=0.3, geometry=geometry,
@@ -210,7 +213,7 @@ def test_step_f32_edge_strips_fast_equals_boundary_path(dev, ybc, method, geomet
     L = native.load()
     old = L.ast_get_tile_rows()
     try:
-        L.ast_set_tile_rows(8)
+        assert L.ast_set_tile_rows(tile) == 0
         a = _run(g, s, u, 3, method, dev, torch.float32, T_do0=0.003)
         L.ast_set_tile_rows(6)
         b = _run(g, s, u, 3, method, dev, torch.float32, T_do0=0.003)
