@@ -295,6 +295,34 @@ def test_sim_graph_equals_eager(dev, precision, runs):
     assert ta["TT"] == TT and ta["i"] == 41
 
 
+def test_sim_tilings_step_alike(dev):
+    """The runtime's graph-replayed FP32 steps give the same bits for every
+    strip tiling: fixed heights (8 rows, 6 rows on the boundary path) and S
+    strips per band (the autotuner's balanced tilings: ends of a multiple of
+    4 rows, the rest with tails of 1-3 rows); a run split by an autotune
+    call (state and time restored) continues to the same bits."""
+    from asteriy_2d_amd.sim import Simulation
+    p = problems.wind(300, 128)
+    res = []
+    for tile in (8, 6, -9, -13, -20):
+        with Simulation(p.grid, G.HLLC, "f32", p.s, p.u) as sim:
+            sim.set_tile_rows(tile)
+            sim.run(23)
+            res.append((sim.state(), sim.time()))
+    for (s, t) in res[1:]:
+        assert np.array_equal(s[0], res[0][0][0]) and np.array_equal(s[1], res[0][0][1])
+        assert t == res[0][1]
+    with Simulation(p.grid, G.HLLC, "f32", p.s, p.u) as sim:
+        sim.run(5)
+        before = (sim.state(), sim.time())
+        tile, us = sim.autotune(reps=3)
+        assert tile != 0 and us > 0
+        assert np.array_equal(sim.state()[0], before[0][0]) and sim.time() == before[1]
+        sim.run(18)
+        s, t = sim.state(), sim.time()
+    assert np.array_equal(s[0], res[0][0][0]) and t == res[0][1]
+
+
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_sim_counting_kernels_change_nothing(dev, precision):
     """The exact-solver kernels compiled with the solver counters step
