@@ -68,7 +68,7 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
     return ast_fail(AST_ERR_ARG, "ast_step_f32: output planes must not alias the input planes");
   if (krows == 0 || krows > 1024 || (krows < 0 && -(long long)krows > g->M))
     return ast_fail(AST_ERR_ARG, "ast_step_f32: strip tiling %d for %d rows", krows, g->M);
-  const int height = krows > 0 ? krows : (g->M - krows - 1) / -krows;  // tallest strip
+  const int height = tallest_strip(g->M, krows);
   KTime kt = {nullptr, nullptr, nullptr, 0};
   if (tm != nullptr) {
     if (tm->nslots < 1 || tm->nslots > kMaxSlots || tm->cur == nullptr)

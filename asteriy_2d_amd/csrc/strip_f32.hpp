@@ -2,6 +2,8 @@
 // single-grid step (step_f32.hip) and the batched persistent launch
 // (batch.hip).  See step_f32.hip for the scheme.
 #pragma once
+#include <algorithm>
+
 #include "common.hpp"
 #include "exact_f32.hpp"
 #include "fast_flux.hpp"
@@ -144,6 +146,15 @@ static inline StripShape strip_shape(const ast_grid_t& g, int krows) {
 }
 // Do the buffer ranges of a strip fit one resource?  (rows krows + 4 of N
 // float2 cells)
+// The tallest strip of a tiling (strip_rows): tile > 0 rows, or for S = -tile
+// strips per band the end strips (M/S rounded to a multiple of 4) or the
+// others (ceil of what is left over them).
+static inline int tallest_strip(int M, int tile) {
+  if (tile > 0) return tile;
+  const int S = -tile, e = std::max(4, (M / S + 2) / 4 * 4), mid = M - 2 * e;
+  if (S >= 3 && mid >= S - 2) return std::max(e, (mid + S - 3) / (S - 2));
+  return (M + S - 1) / S;
+}
 static inline bool strip_fits(const ast_grid_t& g, int krows) {
   return (long long)(krows + 4) * g.N * 8 <= kMaxStripBytes;
 }
