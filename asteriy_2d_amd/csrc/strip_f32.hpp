@@ -273,8 +273,8 @@ struct Row {
 // columns are interior or its band's ghost columns follow a per-lane rule,
 // so rows are raw buffer loads at fixed per-lane offsets (kFast, kFastX: no
 // ghost row, no frozen cell either, rows in groups of four and a tail of 1-3
-// rows; kFastE: ghost rows by row rules, frozen tests per row, a multiple of
-// 4 rows);
+// rows, the exact solver a multiple of 4 rows; kFastE: ghost rows by row
+// rules, frozen tests per row, a multiple of 4 rows);
 // otherwise load() applies the boundary rules, the frozen tests run per row,
 // and the rows of the last group of four that lie past the strip vote for no
 // dt (their stores fall outside the store range).
@@ -483,10 +483,11 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
   };
 
   // kFast, kFastX: whole groups of four, then the 1-3 rows left
-  // (wave-uniform); kFastE strips are whole groups (a tail there costs the
-  // kernel registers); the boundary variant runs its last group whole (rows
-  // past m1 drop out)
-  constexpr bool TAIL = PATH == kFast || PATH == kFastX;
+  // (wave-uniform).  Whole groups only: kFastE strips (a tail there costs
+  // the kernel registers) and the exact solver's strips (a tail would inline
+  // the solver twice more into a kernel whose code size already matters);
+  // the boundary variant runs its last group whole (rows past m1 drop out).
+  constexpr bool TAIL = (PATH == kFast || PATH == kFastX) && !is_exact(METHOD);
   const int mq = TAIL ? m0 + ((m1 - m0) & ~3) : m1;
   for (int m = m0; m < mq; m += 4) {
     row(Row<0>{}, m);
@@ -601,7 +602,8 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
               frozen_possible(g, band * WOUT - 3, band * WOUT + 62, a.m0 - 2, a.m1 + 1);
   if (a.m0 < g.M) {
     // wave-uniform
-    const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 >= 4 && a.m1 - a.m0 <= 64;
+    const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 >= 4 && a.m1 - a.m0 <= 64 &&
+                           (!is_exact(METHOD) || ((a.m1 - a.m0) & 3) == 0);
     const bool rows_in = a.m0 - 2 >= 0 && a.m1 + 1 < g.M;
     if (fast_rows && rows_in && !cw.frozen) {
       if (a.cols_in)
