@@ -144,8 +144,6 @@ static inline StripShape strip_shape(const ast_grid_t& g, int krows) {
   sh.nsg = (nstrips + WPB - 1) / WPB;
   return sh;
 }
-// Do the buffer ranges of a strip fit one resource?  (rows krows + 4 of N
-// float2 cells)
 // The tallest strip of a tiling (strip_rows): tile > 0 rows, or for S = -tile
 // strips per band the end strips (M/S rounded to a multiple of 4) or the
 // others (ceil of what is left over them).
@@ -155,6 +153,8 @@ static inline int tallest_strip(int M, int tile) {
   if (S >= 3 && mid >= S - 2) return std::max(e, (mid + S - 3) / (S - 2));
   return (M + S - 1) / S;
 }
+// Do the buffer ranges of a strip fit one resource?  (rows krows + 4 of N
+// float2 cells)
 static inline bool strip_fits(const ast_grid_t& g, int krows) {
   return (long long)(krows + 4) * g.N * 8 <= kMaxStripBytes;
 }
