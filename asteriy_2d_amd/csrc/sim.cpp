@@ -331,19 +331,44 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   float best_t = 1e30f;
   // the first step of a run: dt from T_do, the learned mode from its word
   const ast::StepTime tm = step_time(S, 0);
+  auto step = [&](int kr) {
+    return S->precision == 1
+               ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
+                                   (float*)u2, S->scal, S->scal + 1, S->method, S->mlp, nullptr,
+                                   kr, &tm, S->stream)
+               : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
+                              (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method, nullptr,
+                              S->stream);
+  };
+  auto valid = [&](int kr) {
+    return S->precision != 1 || !(kr == 0 || kr > 1024 || (kr < 0 && -kr > S->g.M));
+  };
   const int n = S->precision == 1 ? ncand : 1;
+  // warm-up: the clocks of an idle device ramp over the first tens of
+  // milliseconds of work, which would slow the first candidates; untimed
+  // launches of the first valid candidate until 20 ms have passed
   for (int c = 0; c < n; c++) {
     const int kr = S->precision == 1 ? candidates[c] : 0;
-    if (S->precision == 1 && (kr == 0 || kr > 1024 || (kr < 0 && -kr > S->g.M))) continue;
+    if (!valid(kr)) continue;
+    float ms = 0.f;
+    CK(hipEventRecord(e0, S->stream));
+    for (int launched = 0; ms < 20.f && launched < 4000; launched += 10) {
+      for (int r = 0; r < 10; r++) {
+        const int rc = step(kr);
+        if (rc != AST_OK) return rc;
+      }
+      CK(hipEventRecord(e1, S->stream));
+      CK(hipEventSynchronize(e1));
+      CK(hipEventElapsedTime(&ms, e0, e1));
+    }
+    break;
+  }
+  for (int c = 0; c < n; c++) {
+    const int kr = S->precision == 1 ? candidates[c] : 0;
+    if (!valid(kr)) continue;
     for (int r = -2; r < reps; r++) {  // two untimed launches first
       if (r == 0) CK(hipEventRecord(e0, S->stream));
-      int rc = S->precision == 1
-                   ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
-                                       (float*)u2, S->scal, S->scal + 1, S->method, S->mlp,
-                                       nullptr, kr, &tm, S->stream)
-                   : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
-                                  (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
-                                  nullptr, S->stream);
+      const int rc = step(kr);
       if (rc != AST_OK) return rc;
     }
     CK(hipEventRecord(e1, S->stream));

@@ -14,8 +14,9 @@ reference step) on the same workload and the FP32 run's L1 density/pressure
 error against it, together with the FP64 run's own sensitivity floor (FP64
 started from the FP32-rounded initial state); the exact-solver step (FP32
 with FP64 islands vs FP64) at 2048x1024; the exact solver with the learned
-newton start at 4096x2048; the 64-instance batched sweep; the roofline of
-the headline kernel; and the CPU baseline.
+newton start at 4096x2048; the HLLC step at 4096x2048 (its kernel's HBM
+fraction); the 64-instance batched sweep; the roofline of the headline
+kernel; and the CPU baseline.
 
 Roofline: algorithmic bytes of the step kernel (FP32: 16 B read + 16 B
 written per cell-update) over its average launch duration, timed with HIP
@@ -422,6 +423,28 @@ def main(argv=None):
         }
         log(f"exact+learned 4096: {extra['exact_learned_4096x2048']}")
         del c_m, c_64
+
+        # -- the HLLC step on the 4096x2048 grid (4x the cells) --------------
+        ph = problems.wind(4096, 2048, method=HLLC)
+        gh = ph.grid
+        Kh = 200
+        with Simulation(gh, HLLC, "f32", ph.s, ph.u) as sh:
+            rows_h, _ = sh.autotune()
+            mh = _timed(sh, Kh, sh.torch_stream())
+            ch = sh.state()
+            kh_us = sh.autotune([rows_h], reps=100)[1]  # the kernel alone, warm
+        mh64, ch64, _, _, _ = _run_sim(gh, HLLC, "f64", ph.s, ph.u, Kh)
+        extra["hllc_4096x2048"] = {
+            "steps": Kh,
+            "fp32_gcells": round(gh.cells * Kh / mh / 1e6, 3),
+            "fp64_gcells": round(gh.cells * Kh / mh64 / 1e6, 3),
+            "tile_rows": rows_h,
+            "kernel_us": round(kh_us, 2),
+            "hbm_frac": round(32.0 * gh.cells / (kh_us * 1e-6) / 8.0e12, 4),
+            "error_vs_fp64": _errors(ch, ch64),
+        }
+        log(f"hllc 4096: {extra['hllc_4096x2048']}")
+        del ch, ch64
 
         # -- Riemann-flux microbenchmark: 2^24 face pairs --------------------
         extra["flux_2e24"] = _flux_microbenchmark(mdl)
