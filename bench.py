@@ -251,6 +251,9 @@ def main(argv=None):
     ap.add_argument("--rows", type=int, default=0,
                     help="--profile: FP32 strip tiling, rows per strip or -S strips per band "
                          "(default: the autotuner's choice)")
+    ap.add_argument("--grid", default=None,
+                    help="--profile f32 / f64: the wind workload on another grid, NxM "
+                         "(default 2048x1024)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer --steps times (counter calibration)")
     a = ap.parse_args(argv)
@@ -314,13 +317,16 @@ def main(argv=None):
     if a.profile:
         # the HLLC steps as the headline runs them: autotuned strip height,
         # graph replays (FP32: one step kernel per step + one closing fold)
-        sim = Simulation(g, HLLC, a.profile, p.s, p.u)
+        if a.grid:
+            n_, m_ = (int(v) for v in a.grid.lower().split("x"))
+            p = problems.wind(n_, m_, method=HLLC, steps=K)
+        sim = Simulation(p.grid, HLLC, a.profile, p.s, p.u)
         if a.rows:
             sim.set_tile_rows(a.rows)
         else:
             sim.autotune()
         sim.run(K).sync()
-        log(f"step_{a.profile} HLLC {N_GRID}x{M_GRID}: {K} steps")
+        log(f"step_{a.profile} HLLC {p.grid.N}x{p.grid.M}: {K} steps")
         sim.close()
         return 0
 
@@ -441,6 +447,8 @@ def main(argv=None):
             "tile_rows": rows_h,
             "kernel_us": round(kh_us, 2),
             "hbm_frac": round(32.0 * gh.cells / (kh_us * 1e-6) / 8.0e12, 4),
+            "traffic": _traffic("step_f32_hllc_4096x2048_"
+                                + (f"r{rows_h}" if rows_h > 0 else f"s{-rows_h}")),
             "error_vs_fp64": _errors(ch, ch64),
         }
         log(f"hllc 4096: {extra['hllc_4096x2048']}")
