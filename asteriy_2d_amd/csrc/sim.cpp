@@ -299,16 +299,16 @@ extern "C" void* ast_sim_stream(ast_sim_t* S) { return S ? (void*)S->stream : nu
 extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, int reps,
                                 int* best_rows, float* best_us) {
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_autotune: null sim");
-  // default: strips of 4-16 rows, and the tilings that give every SIMD 4 or
+  // default: strips of 4-16 rows, and the tilings that give every SIMD 2 to
   // 5 waves of (nearly) equal height, S strips per band with S * bands
   // waves (a wave strip has 60 output columns; a CU has 4 SIMDs)
-  int dflt[6] = {4, 8, 12, 16};
+  int dflt[8] = {4, 8, 12, 16};
   if (!candidates || ncand <= 0) {
     int nd = 4, dev = 0, cus = 0;
     CK(hipGetDevice(&dev));
     CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int nbands = (S->g.N + 59) / 60;
-    for (int waves = 4; waves <= 5; waves++) {
+    for (int waves = 2; waves <= 5; waves++) {
       const int ns = waves * 4 * cus / nbands;
       if (ns >= 1 && S->g.M / ns >= 4) dflt[nd++] = -ns;
     }

@@ -490,6 +490,17 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
   constexpr bool TAIL = (PATH == kFast || PATH == kFastX) && !is_exact(METHOD);
   const int mq = TAIL ? m0 + ((m1 - m0) & ~3) : m1;
   for (int m = m0; m < mq; m += 4) {
+    if constexpr (FAST && !is_exact(METHOD)) {
+      // wave priority falls with the fraction of the strip done: a SIMD
+      // issues for its oldest wave first, so without this its waves finish
+      // one after another and the last runs alone; with it the waves that
+      // are behind catch up and the SIMD's waves end together
+      const int q = (4 * (m - m0)) / (m1 - m0);  // wave-uniform, 0..3
+      if (q == 0) __builtin_amdgcn_s_setprio(3);
+      else if (q == 1) __builtin_amdgcn_s_setprio(2);
+      else if (q == 2) __builtin_amdgcn_s_setprio(1);
+      else __builtin_amdgcn_s_setprio(0);
+    }
     row(Row<0>{}, m);
     row(Row<1>{}, m + 1);
     row(Row<2>{}, m + 2);

@@ -283,7 +283,7 @@ int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
  * 64 rows inside the grid take the fast strip path. */
 int ast_sim_set_tile_rows(ast_sim_t* sim, int rows);
 /* Times the FP32 step for each candidate tiling (NULL = a default list:
- * 4, 8, 12, 16 rows and the two tilings -S that give every SIMD 4 or 5
+ * 4, 8, 12, 16 rows and the tilings -S that give every SIMD 2, 3, 4 or 5
  * equal waves), keeps the fastest; reports it (may be NULL).  State and
  * time are left unchanged. */
 int ast_sim_autotune(ast_sim_t* sim, const int* candidates, int ncand, int reps,
