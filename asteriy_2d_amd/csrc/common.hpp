@@ -162,19 +162,20 @@ struct KTime {
   int nslots;
 };
 
-// dt of this step.  With a previous slot set, dt is the minimum of its slots
-// (every wave folds them itself: no workgroup waits for another) and wave 0
-// of workgroup 0 performs the previous step's time advance, funk_time of
-// kernel.cu:1525-1536 (T_do = that minimum, TT += T_do, i += 1; T stays at
-// its reset value), and clears the set `rst` for the next step.
-AST_DEV double step_dt(double* T, const double* T_do, const KTime& kt) {
+// dt of this step, in two parts.  step_dt_load (loads only): with a previous
+// slot set, dt is the minimum of its slots (every wave folds them itself: no
+// workgroup waits for another), else *T_do.  step_dt_commit (the writes):
+// with a previous set, wave 0 of workgroup 0 performs the previous step's
+// time advance, funk_time of kernel.cu:1525-1536 (T_do = that minimum,
+// TT += T_do, i += 1; T stays at its reset value); it clears the set `rst`
+// for the step after.
+AST_DEV double step_dt_load(const double* T_do, const KTime& kt) {
   const int lane = threadIdx.x & 63;
-  double t;
-  if (kt.prev != nullptr) {
-    t = wave_min(lane < kt.nslots ? kt.prev[kSlotStride * lane] : INFINITY);
-  } else {
-    t = *T_do;
-  }
+  if (kt.prev != nullptr) return wave_min(lane < kt.nslots ? kt.prev[kSlotStride * lane] : INFINITY);
+  return *T_do;
+}
+AST_DEV void step_dt_commit(double* T, const KTime& kt, double t) {
+  const int lane = threadIdx.x & 63;
   if (blockIdx.x == 0 && threadIdx.x < 64) {
     if (kt.rst != nullptr) {
       if (lane < kt.nslots) kt.rst[kSlotStride * lane] = kSlotEmpty;
@@ -187,7 +188,6 @@ AST_DEV double step_dt(double* T, const double* T_do, const KTime& kt) {
       *i = *i + 1;
     }
   }
-  return t;
 }
 
 // Adds the block's solver counters to *stats (device).

@@ -56,9 +56,13 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   int nwg) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
-  // gated launch: this kernel's mode is not the selected one (grid-uniform)
-  if (ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
-  const double tdo = step_dt(T, T_do, kt);
+  // gated launch: this kernel's mode is not the selected one (grid-uniform);
+  // the gate word is read before the dt slots (it lies beside them), so the
+  // two loads overlap, and tested before anything is written
+  const uint32_t gate = ex.gate != nullptr ? *ex.gate : ex.gate_mode;
+  const double tdo = step_dt_load(T_do, kt);
+  if (gate != ex.gate_mode) return;
+  step_dt_commit(T, kt, tdo);
   constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
   typename std::conditional<STATS, StatsF32,
                             typename std::conditional<kDetects, StrongStats, NoStats>::type>::type st;
