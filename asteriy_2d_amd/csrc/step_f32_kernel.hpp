@@ -56,9 +56,12 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   int nwg) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
-  // gated launch: this kernel's mode is not the selected one (grid-uniform);
-  // the gate word is read before the dt slots (it lies beside them), so the
-  // two loads overlap, and tested before anything is written
+  // gated launch: this kernel's mode is not the selected one (grid-uniform).
+  // The learned kernel, idle unless the previous step had strong faces, tests
+  // its gate first; the others read the gate word before the dt slots (it
+  // lies beside them), so the two loads overlap, and test it before anything
+  // is written
+  if (METHOD == kExactLearned && ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
   const uint32_t gate = ex.gate != nullptr ? *ex.gate : ex.gate_mode;
   const double tdo = step_dt_load(T_do, kt);
   if (gate != ex.gate_mode) return;
