@@ -221,6 +221,12 @@ def _cpu_baseline(grid, s, u, method, budget_s: float):
                       f"by the C oracle ({el:.1f} s, {threads} OpenMP threads)"}
 
 
+def _tiling(tile: int) -> str:
+    """The FP32 strip tiling in words (tile > 0: rows per strip; -S: S strips
+    of nearly equal height per 60-column band)."""
+    return f"{tile} rows per strip" if tile > 0 else f"{-tile} strips per band"
+
+
 def _traffic(name: str):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel), recorded in profiles/."""
@@ -359,6 +365,7 @@ def main(argv=None):
         "error_vs_fp64": _errors(r32, r64),
         "sim_time": {"fp32": t32["TT"], "fp64": t64["TT"]},
         "tile_rows": rows,
+        "strip_tiling": _tiling(rows),
         "kernel_us": {"step_f32_hllc": round(kms32 * 1e3, 2),
                       "step_f64_hllc": round(kms64 * 1e3, 2)},
         "fp64_hbm_frac": round(64.0 * cells / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -445,6 +452,7 @@ def main(argv=None):
             "fp32_gcells": round(gh.cells * Kh / mh / 1e6, 3),
             "fp64_gcells": round(gh.cells * Kh / mh64 / 1e6, 3),
             "tile_rows": rows_h,
+            "strip_tiling": _tiling(rows_h),
             "kernel_us": round(kh_us, 2),
             "hbm_frac": round(32.0 * gh.cells / (kh_us * 1e-6) / 8.0e12, 4),
             "traffic": _traffic("step_f32_hllc_4096x2048_"
