@@ -42,6 +42,7 @@ SOURCES = [
     ("step_f32_hll.hip", FAST),
     ("step_f32_exact.hip", FAST),
     ("step_f32_learned.hip", FAST),
+    ("step_f32_detect.hip", FAST),
     ("mlp.hip", FAST),
     ("batch.hip", FAST),
 ]

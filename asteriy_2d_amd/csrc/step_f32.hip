@@ -29,8 +29,8 @@
 //    guess (exact_f32.hpp).
 //
 // This source holds the host entry point; the kernels are instantiated per
-// method in step_f32_hll.hip, step_f32_exact.hip and step_f32_learned.hip
-// (step_f32_kernel.hpp).
+// method in step_f32_hll.hip, step_f32_exact.hip, step_f32_learned.hip and
+// step_f32_detect.hip (step_f32_kernel.hpp).
 #include "step_f32_kernel.hpp"
 
 using namespace ast;
