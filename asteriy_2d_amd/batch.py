@@ -22,6 +22,8 @@ BATCH_SIGNATURES = {
     "ast_batch_set_state": (_c_int, [_vp, _c_int, _dp, _dp]),
     "ast_batch_get_state": (_c_int, [_vp, _c_int, _dp, _dp]),
     "ast_batch_get_time": (_c_int, [_vp, _c_int, _dp, _dp, _dp, ctypes.POINTER(_c_int)]),
+    "ast_batch_set_time": (_c_int, [_vp, _c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, _c_int]),
     "ast_batch_autotune": (_c_int, [_vp, _c_int]),
     "ast_batch_run": (_c_int, [_vp, _c_int, _c_int]),
     "ast_batch_sync": (_c_int, [_vp]),
@@ -90,6 +92,9 @@ class Batch:
         native.call("ast_batch_get_time", self.h, int(k), ctypes.byref(T), ctypes.byref(Td),
                     ctypes.byref(TT), ctypes.byref(i))
         return {"T": T.value, "T_do": Td.value, "TT": TT.value, "i": i.value}
+
+    def set_time(self, k, T=1e7, T_do=1e-9, TT=0.0, i=0):
+        native.call("ast_batch_set_time", self.h, int(k), T, T_do, TT, int(i))
 
     def autotune(self, reps: int = 5):
         native.call("ast_batch_autotune", self.h, int(reps))

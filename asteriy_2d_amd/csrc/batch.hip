@@ -266,6 +266,12 @@ extern "C" int ast_batch_set_state(ast_batch_t* B, int k, const double* s, const
     a[q] = (float)s[q];
     b[q] = (float)u[q];
   }
+  // the FP32 strip reconstructs without the positivity fallback: every cell
+  // needs rho > 0 and p > 0 (as ast_sim_set_state checks)
+  for (size_t q = 0; q < n; q++)
+    if (!(a[q] > 0.f))
+      return ast_fail(AST_ERR_ARG, "ast_batch_set_state: instance %d cell %zu has %s %g (the FP32 "
+                      "step needs rho > 0 and p > 0)", k, q / 2, (q & 1) ? "p" : "rho", (double)a[q]);
   AST_HIP_CHECK(hipMemcpy(I.plane[2 * B->cur], a.data(), n * sizeof(float), hipMemcpyHostToDevice));
   AST_HIP_CHECK(hipMemcpy(I.plane[2 * B->cur + 1], b.data(), n * sizeof(float), hipMemcpyHostToDevice));
   return AST_OK;
@@ -299,6 +305,16 @@ extern "C" int ast_batch_get_time(ast_batch_t* B, int k, double* T, double* T_do
   if (T_do) *T_do = v[1];
   if (TT) *TT = v[2];
   if (i) memcpy(i, &v[3], sizeof *i);
+  return AST_OK;
+}
+
+extern "C" int ast_batch_set_time(ast_batch_t* B, int k, double T, double T_do, double TT, int i) {
+  int rc = batch_check(B, k, "ast_batch_set_time");
+  if (rc != AST_OK) return rc;
+  double v[4] = {T, T_do, TT, 0.0};
+  memcpy(&v[3], &i, sizeof i);
+  AST_HIP_CHECK(hipStreamSynchronize(B->stream));
+  AST_HIP_CHECK(hipMemcpy(B->h[k].scal, v, sizeof v, hipMemcpyHostToDevice));
   return AST_OK;
 }
 

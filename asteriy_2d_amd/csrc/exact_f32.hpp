@@ -193,7 +193,13 @@ AST_DEV float newton_f32(const ExactCtx& e, float enI, float pI, float rI, float
   const float zI = rI * cI, zII = rII * cII;
   float pn = (pI * zII + pII * zI + (enI - enII) * zI * zII) * frcp(zI + zII);
   const float pee = pn;
-  if (use_guess) pn = guess;
+  // a learned start is clamped into the two-shock range p* > pII: the
+  // function is concave and increasing in p, so Newton from at or below the
+  // root climbs to it monotonically, while a start far above the root would
+  // jump below zero (and sqrt of a negative argument is NaN); every iterate
+  // is clamped as well
+  const float plo = use_guess ? pII : -INFINITY;
+  if (use_guess) pn = fmaxf(guess, pII);
   const float ipI = frcp(pI), ipII = frcp(pII);
   const float k1 = g + 1.f, k3 = 3.f * g - 1.f, k4 = 4.f * g;
   int k = 0;
@@ -207,7 +213,7 @@ AST_DEV float newton_f32(const ExactCtx& e, float enI, float pI, float rI, float
     const float fII = (p - pII) * frcp(zII * sII);
     const float dI = (k1 * p * ipI + k3) * frcp(k4 * zI * aI * sI);
     const float dII = (k1 * p * ipII + k3) * frcp(k4 * zII * aII * sII);
-    pn = p - (fI + fII - (enI - enII)) * frcp(dI + dII);
+    pn = fmaxf(p - (fI + fII - (enI - enII)) * frcp(dI + dII), plo);
     // the reference's rule |pn - p| < tol*pee, floored at the FP32
     // resolution of the residual (pee can lie far below p* in strong
     // shocks, where the residual's rounding moves p by ~2 ulp); close to

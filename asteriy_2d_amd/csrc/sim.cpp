@@ -606,6 +606,13 @@ extern "C" int ast_sim_load(ast_sim_t* S, const char* path) {
   ok = ok && fread(a.data(), 1, bytes, f) == bytes && fread(b.data(), 1, bytes, f) == bytes;
   fclose(f);
   if (!ok) return ast_fail(AST_ERR_ARG, "ast_sim_load: %s is truncated", path);
+  if (S->precision == 1) {  // the FP32 step needs rho > 0 and p > 0 in every cell
+    const float* sv = reinterpret_cast<const float*>(a.data());
+    for (size_t k = 0; k < S->cells * 2; k++)
+      if (!(sv[k] > 0.f))
+        return ast_fail(AST_ERR_ARG, "ast_sim_load: %s: cell %zu has %s %g (the FP32 step needs "
+                        "rho > 0 and p > 0)", path, k / 2, (k & 1) ? "p" : "rho", (double)sv[k]);
+  }
   CK(hipStreamSynchronize(S->stream));
   S->cur = 0;
   CK(hipMemcpy(S->plane[0], a.data(), bytes, hipMemcpyHostToDevice));

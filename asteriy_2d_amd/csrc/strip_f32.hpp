@@ -458,15 +458,18 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
       dmy -= tr * q0.vy;
       dE -= ty * q0.vy * (c.gig1 * q0.p + ke);
     }
-    float r2 = q0.r + dr_;
-    r2 = (r2 <= 0.f) ? q0.r : r2;  // kernel.cu:3060-3064
+    // kernel.cu:3060-3064: a non-positive density keeps the old one; the
+    // density change that enters the kinetic energy below is then zero
+    const bool rfloor = q0.r + dr_ <= 0.f;
+    const float drf = rfloor ? 0.f : dr_;
+    float r2 = q0.r + drf;
     const float ir = frcp(r2);
     float vx2 = (mx + dmx) * ir;
     float vy2 = (my + dmy) * ir;
     // p' = (E' - rho'|u'|^2/2)(gamma-1), written so that the O(E) terms
     // cancel exactly: E' - KE' = p/(g-1) + dE - (KE' - KE) with
-    // KE' - KE = (m.dm + |dm|^2/2 - KE*dr) / rho'.
-    const float dke = (mx * dmx + my * dmy + 0.5f * (dmx * dmx + dmy * dmy) - ke * dr_) * ir;
+    // KE' - KE = (m.dm + |dm|^2/2 - KE*dr) / rho' (dr = rho' - rho).
+    const float dke = (mx * dmx + my * dmy + 0.5f * (dmx * dmx + dmy * dmy) - ke * drf) * ir;
     float p2 = q0.p + c.g1 * (dE - dke);
     p2 = (p2 <= 0.f) ? 0.000001f : p2;  // kernel.cu:3070-3073
     if (frz && fr[R::e0]) {  // kernel.cu:2317-2324

@@ -322,6 +322,8 @@ int ast_batch_destroy(ast_batch_t* b);
 int ast_batch_set_state(ast_batch_t* b, int inst, const double* s, const double* u);
 int ast_batch_get_state(ast_batch_t* b, int inst, double* s, double* u);
 int ast_batch_get_time(ast_batch_t* b, int inst, double* T, double* T_do, double* TT, int* i);
+/* Sets one instance's time scalars (as ast_sim_set_time). */
+int ast_batch_set_time(ast_batch_t* b, int inst, double T, double T_do, double TT, int i);
 /* Per-shape strip heights: times candidate heights on the batch itself. */
 int ast_batch_autotune(ast_batch_t* b, int reps);
 int ast_batch_run(ast_batch_t* b, int steps, int use_graph);

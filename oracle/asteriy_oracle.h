@@ -67,6 +67,9 @@ int toro_star(double gamma, double dl, double ul, double pl, double dr,
 void toro_sample(double gamma, double dl, double ul, double pl, double dr,
                  double ur, double pr, double pm, double um, double S,
                  double* d, double* u, double* p);
+int toro_sample_vacuum(double gamma, double dl, double ul, double pl,
+                       double dr, double ur, double pr, double S, double* d,
+                       double* u, double* p, double* speeds);
 void toro_sod_profile(double gamma, double dl, double ul, double pl, double dr,
                       double ur, double pr, double x0, double t, long n,
                       const double* x, double* d, double* u, double* p);

@@ -38,6 +38,35 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+_HOST_BUILDS = (("x86-64-v4", "liboracle_v4.so", ("avx512f", "avx512bw", "avx512dq", "avx512vl")),
+                ("x86-64-v3", "liboracle_v3.so", ("avx2", "fma", "bmi2", "movbe")),
+                ("x86-64-v2", "liboracle.so", ()))
+
+
+def select_host_build() -> str:
+    """Loads the oracle build for the widest instruction set this host's
+    /proc/cpuinfo lists (for the timed CPU baseline; every build gives the
+    same bits) and returns its -march level."""
+    global _lib, LIB_PATH
+    try:
+        flags = set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+    except OSError:
+        flags = set()
+    for march, name, need in _HOST_BUILDS:
+        path = os.path.join(HERE, name)
+        if all(f in flags for f in need) and os.path.exists(path):
+            if path != LIB_PATH:
+                LIB_PATH = path
+                _lib = None
+            lib()
+            return march
+    return "x86-64-v2"
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -75,6 +104,8 @@ def lib():
         L.toro_star.restype = ctypes.c_int
         L.toro_star.argtypes = [ctypes.c_double] * 7 + [_dp, _dp, _ip]
         L.toro_sample.argtypes = [ctypes.c_double] * 10 + [_dp, _dp, _dp]
+        L.toro_sample_vacuum.restype = ctypes.c_int
+        L.toro_sample_vacuum.argtypes = [ctypes.c_double] * 8 + [_dp, _dp, _dp, _dp]
         L.toro_sod_profile.argtypes = [ctypes.c_double] * 9 + [ctypes.c_long, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
@@ -223,6 +254,19 @@ def toro_sample(gamma, dl, ul, pl, dr, ur, pr, S):
     lib().toro_sample(gamma, dl, ul, pl, dr, ur, pr, st[0], st[1], S, ctypes.byref(d),
                       ctypes.byref(u), ctypes.byref(p))
     return d.value, u.value, p.value
+
+
+def toro_sample_vacuum(gamma, dl, ul, pl, dr, ur, pr, S):
+    """Toro's vacuum-generation solution (sect. 4.6) at x/t = S: (d, u, p,
+    (left head, left vacuum front, right vacuum front, right head)), or None
+    when the data do not generate vacuum."""
+    d, u, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    sp = np.empty(4)
+    rc = lib().toro_sample_vacuum(gamma, dl, ul, pl, dr, ur, pr, S, ctypes.byref(d),
+                                  ctypes.byref(u), ctypes.byref(p), _p(sp))
+    if rc != 0:
+        return None
+    return d.value, u.value, p.value, tuple(sp)
 
 
 def toro_profile(gamma, left, right, x0, t, x):

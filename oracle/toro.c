@@ -122,6 +122,43 @@ void toro_sample(double g, double dl, double ul, double pl, double dr,
   }
 }
 
+/* Solution sampled at speed S = x/t when the data generate vacuum
+ * (Toro sect. 4.6, 2(cL + cR)/(g - 1) <= uR - uL): a left rarefaction from
+ * its head uL - cL to the vacuum front uL + 2 cL/(g - 1), vacuum (d = p = 0,
+ * u = S), a right rarefaction from its front uR - 2 cR/(g - 1) to its head
+ * uR + cR.  The fans follow the rarefaction relations of sect. 4.5
+ * (eqs. 4.56 and 4.63).  Returns 0, or -1 when the data do not generate
+ * vacuum. */
+int toro_sample_vacuum(double g, double dl, double ul, double pl, double dr,
+                       double ur, double pr, double S, double* d, double* u,
+                       double* p, double* speeds) {
+  double cl = sqrt(g * pl / dl), cr = sqrt(g * pr / dr);
+  double g4 = 2.0 / (g - 1.0), g5 = 2.0 / (g + 1.0), g6 = (g - 1.0) / (g + 1.0);
+  double g7 = (g - 1.0) / 2.0;
+  if (g4 * (cl + cr) > ur - ul) return -1;
+  double shl = ul - cl, sfl = ul + g4 * cl; /* left head, left vacuum front  */
+  double sfr = ur - g4 * cr, shr = ur + cr; /* right vacuum front, right head */
+  if (speeds) { speeds[0] = shl; speeds[1] = sfl; speeds[2] = sfr; speeds[3] = shr; }
+  if (S <= shl) { *d = dl; *u = ul; *p = pl; return 0; }
+  if (S < sfl) {
+    double q = g5 + g6 / cl * (ul - S);
+    *d = dl * pow(q, g4);
+    *u = g5 * (cl + g7 * ul + S);
+    *p = pl * pow(q, 2.0 * g / (g - 1.0));
+    return 0;
+  }
+  if (S <= sfr) { *d = 0.0; *u = S; *p = 0.0; return 0; }
+  if (S < shr) {
+    double q = g5 - g6 / cr * (ur - S);
+    *d = dr * pow(q, g4);
+    *u = g5 * (-cr + g7 * ur + S);
+    *p = pr * pow(q, 2.0 * g / (g - 1.0));
+    return 0;
+  }
+  *d = dr; *u = ur; *p = pr;
+  return 0;
+}
+
 /* Exact solution of a shock-tube problem at time t on points x
  * (discontinuity at x0). */
 void toro_sod_profile(double g, double dl, double ul, double pl, double dr,

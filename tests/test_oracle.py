@@ -386,3 +386,175 @@ def test_frozen_cells_copied_and_silent(ora):
     assert fr.sum() > 0
     assert np.array_equal(s2[fr], s[fr]) and np.array_equal(u2[fr], u[fr])
     assert not np.array_equal(s2[~fr], s[~fr])
+
+
+# ---- the vacuum branch against Toro's vacuum solution (kernel.cu:480-544) -------
+
+def _vacuum_faces(g):
+    """Faces that generate vacuum, chosen so that x/t = 0 falls in each of the
+    five regions of the solution (left state, left fan, vacuum, right fan,
+    right state), plus the near-vacuum pairs of the microbenchmark generator
+    that do generate vacuum."""
+    k = 2.0 / (g - 1.0)
+    out = []
+    for rl, pl, rr, pr in ((1.0, 1.0, 1.0, 1.0), (0.3, 2.0, 4.0, 0.01), (2.0, 0.05, 0.1, 3.0)):
+        cl, cr = math.sqrt(g * pl / rl), math.sqrt(g * pr / rr)
+        # uR - uL = 1.2 * 2(cL + cR)/(g - 1): vacuum
+        du = 1.2 * k * (cl + cr)
+        for ul in (1.5 * cl,                 # left head to the right of the face
+                   -0.5 * k * cl,            # face inside the left fan
+                   -1.05 * k * cl,           # face in the vacuum
+                   -du + 0.5 * k * cr,       # face inside the right fan
+                   -du - 1.5 * cr):          # right head to the left of the face
+            out.append(((rl, ul, pl, 0.3), (rr, ul + du, pr, -0.2)))
+    F = facegen.faces(20000, gamma=g, seed=17)
+    for i in range(F.shape[1]):
+        L = (F[0, i], F[2, i], F[1, i], F[3, i])
+        R = (F[4, i], F[6, i], F[5, i], F[7, i])
+        cl, cr = math.sqrt(g * L[2] / L[0]), math.sqrt(g * R[2] / R[0])
+        if k * (cl + cr) < R[1] - L[1]:
+            out.append((L, R))
+    return out
+
+
+def test_vacuum_branch_matches_toro_vacuum_solution(ora):
+    """Faces whose data generate vacuum (2(cL + cR)/(g - 1) < uR - uL): the
+    oracle's flux and dt equal the flux of Toro's vacuum solution sampled at
+    x/t = 0 and krit*rad over the largest of its four wave speeds.  Covers
+    every region of the solution."""
+    g, krit, rad = 5.0 / 3.0, 0.3, 0.2
+    regions = set()
+    faces = _vacuum_faces(g)
+    assert len(faces) > 100
+    for (rl, ul, pl, vl), (rr, ur, pr, vr) in faces:
+        Ps, Pu, t, info = ora.godunov_info([rl, pl], [ul, vl], [rr, pr], [ur, vr], gamma=g,
+                                           krit=krit, rad=rad)
+        assert info.pattern == -1
+        d, u, p, sp = ora.toro_sample_vacuum(g, rl, ul, pl, rr, ur, pr, 0.0)
+        regions.add(sum(0.0 > s for s in sp))  # 0..4: which wave lies left of the face
+        v = vl if u > 0 else vr
+        if d == 0.0:
+            v = 0.0
+        rs, ru = phys_flux(d, p, u, v, 1.0, 0.0, g)
+        ref = np.concatenate([rs, ru])
+        got = np.concatenate([Ps, Pu])
+        scale = max(rl * (abs(ul) + 1) ** 2 + pl, rr * (abs(ur) + 1) ** 2 + pr) * \
+            (1 + abs(ul) + abs(ur) + math.sqrt(g * pl / rl) + math.sqrt(g * pr / rr))
+        assert np.abs(got - ref).max() <= 1e-12 * scale, ((rl, ul, pl), (rr, ur, pr), got, ref)
+        assert abs(t - krit * rad / max(abs(s) for s in sp)) <= 1e-14 * t
+    assert regions == {0, 1, 2, 3, 4}
+
+
+# ---- the axisymmetric source with v != 0 (KorolkovSolver.cu:13-32) ---------------
+
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
+def test_axisymmetric_source_closed_form(ora, method):
+    """A uniform state (rho, p, u, v) with zero-gradient ghosts on every side
+    has equal fluxes through every face, so one axisymmetric step changes each
+    cell only by the r-source -(T_do/y) v {rho, rho u, rho v, gamma p/(g-1) +
+    rho|u|^2/2} at the cell-centre radius y.  Worked through: rho' = rho (1 -
+    T_do v/y), u' = u, v' = v, p' = p (1 - gamma T_do v/y).  A wrong sign, a
+    dropped factor gamma, the kinetic term left out of the energy source or a
+    node radius in place of the centre radius all fail."""
+    g = G.Grid.from_extent(12, 9, -1.0, 1.0, 0.9, gamma=5.0 / 3.0, krit=0.3,
+                           geometry=G.AXISYMMETRIC, bc=(G.BC_OUTFLOW,) * 4)
+    rho, p, ux, uy = 0.8, 1.3, 0.45, -0.7
+    s = np.empty((g.M, g.N, 2))
+    s[..., 0], s[..., 1] = rho, p
+    u = np.empty_like(s)
+    u[..., 0], u[..., 1] = ux, uy
+    T_do = 0.01
+    s2, u2, _ = ora.step(g, s, u, 1e7, T_do, method)
+    y = g.y(np.arange(g.M))[:, None] * np.ones((1, g.N))
+    assert abs(g.y(0) - 0.05) < 1e-15  # centre radius y_min + m dy, y_min = dy/2
+    f = T_do * uy / y
+    assert np.abs(s2[..., 0] - rho * (1 - f)).max() <= 1e-14
+    assert np.abs(s2[..., 1] - p * (1 - g.gamma * f)).max() <= 1e-14
+    assert np.abs(u2[..., 0] - ux).max() <= 1e-14
+    assert np.abs(u2[..., 1] - uy).max() <= 1e-14
+    # the source really acted (v != 0): the change is not round-off
+    assert np.abs(s2[..., 0] - rho).min() > 1e-4
+
+
+# ---- the floors of the update (kernel.cu:3060-3073) ------------------------------
+
+def floor_case(T_do, w=0.6, a=2.0, gamma=5.0 / 3.0, geometry=G.PLANAR):
+    """A 5x2 planar grid whose middle column (n = 2) sits at rest between
+    columns moving away from it at speed a, every cell with rho = p = 1 and
+    tangential velocity w, all faces first order (HLL on the cell states,
+    kernel.cu:2812, 2856).  The rows are equal and the y ghosts copy the edge
+    rows, so the y fluxes cancel.  Returns (grid, s, u, expected middle cell
+    (rho', p', vx', vy') by the closed form below)."""
+    g = G.Grid.from_extent(5, 2, 0.0, 0.4, 0.2, gamma=gamma, krit=0.3, geometry=geometry,
+                           bc=(G.BC_OUTFLOW,) * 4, ho_lo=0.0, ho_hi=0.0)
+    s = np.ones((2, 5, 2))
+    u = np.zeros((2, 5, 2))
+    u[:, :2, 0], u[:, 3:, 0] = -a, a
+    u[..., 1] = w
+    # HLL with Davis speeds at the east face of the middle cell: SL = -c,
+    # SR = a + c; its mass flux c a/(a + 2c), tangential momentum c a w/(a + 2c),
+    # energy c ((E_R + p) a - (a + c) a^2/2)/(a + 2c) with E_R = p/(g-1) +
+    # (a^2 + w^2)/2 (the w^2 parts of E_R - E_L cancel); the west face is its
+    # mirror image (mass and energy leave through it as well, the normal
+    # momentum flux is even and cancels)
+    c = math.sqrt(gamma)
+    E0 = 1.0 / (gamma - 1) + 0.5 * w * w
+    ER = 1.0 / (gamma - 1) + 0.5 * (a * a + w * w)
+    k = T_do / g.dx
+    dr = -2 * k * c * a / (a + 2 * c)
+    dE = -2 * k * c * ((ER + 1.0) * a - (a + c) * a * a / 2) / (a + 2 * c)
+    dmy = w * dr
+    r2 = 1.0 + dr
+    if r2 <= 0:
+        r2 = 1.0                    # kernel.cu:3060-3064: keep the old density
+    vy = (w + dmy) / r2
+    p2 = (E0 + dE - 0.5 * r2 * vy * vy) * (gamma - 1)
+    if p2 <= 0:
+        p2 = 1e-6                   # kernel.cu:3070-3073
+    return g, s, u, (r2, p2, 0.0, vy)
+
+
+# T_do of the three regimes of floor_case: no floor, the density floor alone,
+# both floors
+FLOOR_REGIMES = [(0.005, False, False), (0.1, True, False), (0.3, True, True)]
+
+
+@pytest.mark.parametrize("T_do,rho_floor,p_floor", FLOOR_REGIMES)
+def test_update_floors_closed_form(ora, T_do, rho_floor, p_floor):
+    """The middle cell of floor_case: no floor, the density floor alone (then
+    the kinetic energy of the new state uses the kept density), and both
+    floors (p' = 1e-6 exactly, rho' = rho exactly)."""
+    g, s, u, want = floor_case(T_do)
+    assert (want[0] == 1.0) == rho_floor and (want[1] == 1e-6) == p_floor
+    s2, u2, _ = ora.step(g, s, u, 1e7, T_do, G.HLLC)
+    got = (s2[0, 2, 0], s2[0, 2, 1], u2[0, 2, 0], u2[0, 2, 1])
+    assert (got[0] == 1.0) == rho_floor and (got[1] == 1e-6) == p_floor
+    for k in range(4):
+        assert abs(got[k] - want[k]) <= 1e-13 * (abs(want[k]) + 1), (k, got, want)
+    assert np.array_equal(s2[0], s2[1]) and np.array_equal(u2[0], u2[1])
+
+
+# ---- failure handling: capped loops and NaN input (kernel.cu:647-675, 710-740) ----
+
+def test_devtwo_with_nan_residual_is_capped_not_endless(ora):
+    """A face with zero pressure on one side at rest is a shock + rarefaction
+    pattern whose bracket function lev (kernel.cu:751-765) is NaN everywhere
+    (0/0 in the shock branch): the reference's bisection never meets its
+    |f| <= 1e-9 stop and loops forever (kernel.cu:710-740).  The oracle stops
+    at the cap, flags it and returns."""
+    import time
+    t0 = time.perf_counter()
+    _, _, _, info = ora.godunov_info([1.0, 0.0], [0.0, 0.0], [1.0, 1.0], [0.0, 0.0])
+    assert info.pattern == 1
+    assert info.capped == 1 and info.iters == G.ITER_CAP
+    assert time.perf_counter() - t0 < 5.0
+
+
+def test_nan_input_face_returns_nan_without_iterating(ora):
+    """NaN input matches no wave pattern: the oracle returns NaN fluxes and
+    does not iterate (the reference reads uninitialised variables there)."""
+    for L, R in ((([float("nan"), 1.0], [0.0, 0.0]), ([1.0, 1.0], [0.0, 0.0])),
+                 (([1.0, 1.0], [float("nan"), 0.0]), ([1.0, 1.0], [0.0, 0.0]))):
+        Ps, Pu, _, info = ora.godunov_info(L[0], L[1], R[0], R[1])
+        assert info.pattern == -9 and info.iters == 0
+        assert np.isnan(Ps).all() and np.isnan(Pu).all()
