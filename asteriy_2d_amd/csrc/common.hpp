@@ -1,13 +1,13 @@
 // common.hpp -- shared device/host helpers of the HIP kernels: error
 // reporting of the C-ABI, the XCD-aware tile order, the ghost-cell rules,
-// and the deterministic block reductions (dt min, solver counters).
+// and the deterministic block reductions (dt min).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
 #include "../../include/asteriy.h"
-#include "ref_flux.hpp"
+#include "dev.hpp"
 #include "time_slots.h"
 
 // ---- host side: status + message of the last failure (api.cpp) ----------
@@ -106,6 +106,11 @@ AST_DEV double wave_min(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
   return v;
 }
+AST_DEV double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
 AST_DEV uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -187,26 +192,6 @@ AST_DEV void step_dt_commit(double* T, const KTime& kt, double t) {
       int* i = reinterpret_cast<int*>(T + 3);
       *i = *i + 1;
     }
-  }
-}
-
-// Adds the block's solver counters to *stats (device).
-template <int NT>
-AST_DEV void block_stats_to(const FaceStats& st, uint32_t* red, ast_stats_t* stats) {
-  constexpr int NW = NT / 64;
-  uint32_t v[10] = {st.exact, st.newton, st.newton_it, st.bisect, st.bisect_it,
-                    st.raref, st.vacuum, st.capped, st.fp64, st.mlp};
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < 10; i++) {
-    uint32_t w = wave_sum(v[i]);
-    if (lane == 0) red[wave * 10 + i] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x < 10) {
-    unsigned long long t = 0;
-    for (int w = 0; w < NW; w++) t += red[w * 10 + threadIdx.x];
-    if (t) atomicAdd(reinterpret_cast<unsigned long long*>(stats) + threadIdx.x, t);
   }
 }
 

@@ -29,7 +29,7 @@
 
 #include "fast_flux.hpp"
 #include "mlp.hpp"
-#include "ref_flux.hpp"
+#include "dev.hpp"
 
 namespace ast {
 
@@ -106,6 +106,7 @@ AST_DEV float pow_igm(const ExactCtx& e, float x) { return e.n ? ipow(x, e.n) : 
 // rarefactions, 1 shock + rarefaction, 2 two shocks; kNoFace: not an exact
 // face, e.g. a low-order face of the window).
 constexpr int kNoFace = -3;
+constexpr int kNoPattern = -9;  // an exact face whose input matches no wave pattern (NaN)
 struct FaceInfo {
   int pat, it;
   bool capped, island, learned;
@@ -336,9 +337,10 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   float p = 0.f, t = 1.f;
   int it = 0;
   bool capped = false, island = false, learned = false;
+  // each pattern's whole interval is tested, so NaN input matches none
   if (Udf < Uvk) {
     pat = -1;
-  } else if (Udf <= Urz) {
+  } else if (Udf >= Uvk && Udf <= Urz) {
     // two rarefactions, Urz - Uvk = 2/(g-1) (cII q + cI) (no cancellation)
     pat = 0;
     const float den = e.c2g1 * (cII * q + cI);
@@ -351,7 +353,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
       island = true;
     }
     p = pI * pow_igm(e, t);
-  } else if (Udf <= Uud) {
+  } else if (Udf > Urz && Udf <= Uud) {
     pat = 1;
     float D;
     if (pI < kRarefIsland * pII) {  // FP64 island: strong rarefaction
@@ -363,11 +365,13 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
     }
     p = shock_raref<float>(g, g1, e.gp, gm, enI, pI, rI, enII, pII, rII, cI, cII, D, kNewtonTol32,
                            it, capped);
-  } else {
+  } else if (Udf > Uud) {
     pat = 2;
     if constexpr (DETECT && !LEARNED) strong = strong_pvrs();
     learned = guess > 0.f && guess < 3.0e38f;
     p = newton_f32(e, enI, pI, rI, cI, enII, pII, rII, cII, guess, learned, it, capped);
+  } else {
+    pat = kNoPattern;  // NaN input
   }
   fi.pat = pat;
   fi.it = it;
@@ -437,7 +441,7 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
       const float ce = -e.g1g2 * enII + e.c2g2 * cII;
       en = -ce; pp = pII * pow_igm(e, ce * frcp(cII)); r = g * pp * frcp(ce * ce); te = teII;
     } else { en = enII; pp = pII; r = rII; te = teII; }
-  } else {  // vacuum, kernel.cu:480-544
+  } else if (pat == -1) {  // vacuum, kernel.cu:480-544
     const float dI = enI - cI, ddI = enI + e.c2g1 * cI;
     const float dII = enII + cII, ddII = enII - e.c2g1 * cII;
     UU = fmaxf(fmaxf(fabsf(dI), fabsf(dII)), fmaxf(fabsf(ddII), fabsf(ddI)));
@@ -450,6 +454,9 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
       const float ce = -e.g1g2 * enII + e.c2g2 * cII;
       en = -ce; pp = pII * pow_igm(e, ce * frcp(cII)); r = g * pp * frcp(ce * ce); te = teII;
     } else { en = enII; pp = pII; r = rII; te = teII; }
+  } else {  // no pattern (NaN input): NaN fluxes, dt 0
+    UU = INFINITY;
+    en = pp = r = te = __int_as_float(0x7fc00000);
   }
   if (!(UU > 1e-7f)) UU = 1e-7f;  // eps8 floor of the reference's dt, kernel.cu:246-253
   if (sw) en = -en;

@@ -13,7 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "ref_flux.hpp"
+#include "dev.hpp"
 
 namespace ast {
 

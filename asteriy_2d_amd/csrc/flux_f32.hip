@@ -52,7 +52,8 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
       st.add(fi);
       if (info && valid) {
         info[2 * i] = fi.it;
-        info[2 * i + 1] = (fi.pat + 1) | (fi.capped ? 8 : 0) | (fi.island ? 16 : 0);
+        info[2 * i + 1] = (fi.pat == kNoPattern ? -1 : fi.pat + 1) | (fi.capped ? 8 : 0) |
+                          (fi.island ? 16 : 0);
       }
     }
     if (valid) {

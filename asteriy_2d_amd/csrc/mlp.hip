@@ -12,6 +12,7 @@
 //    kernel (deterministic).
 //  * Adam on the FP32 masters, then the BF16 MFMA operands are rebuilt.
 #include "common.hpp"
+#include "flux64.hpp"
 #include "mlp.hpp"
 
 namespace ast {
@@ -19,7 +20,7 @@ namespace mlp {
 
 // ---- features from the FP64 exact solve ---------------------------------
 __global__ void features_kernel(size_t n, const double* __restrict__ in, double n1, double n2,
-                                Phys<double> ph, float* __restrict__ feats,
+                                d64::GasD gas, float* __restrict__ feats,
                                 float* __restrict__ target, unsigned char* __restrict__ mask) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -31,19 +32,20 @@ __global__ void features_kernel(size_t n, const double* __restrict__ in, double 
     const double enI = sw ? -uR : uL, enII = sw ? -uL : uR;
     const double pI = sw ? pR : pL, pII = sw ? pL : pR, rI = sw ? rR : rL, rII = sw ? rL : rR;
     float f[IN];
-    const float lp = features<double>(rI, pI, enI, rII, pII, enII, ph.ga, f);
+    const float lp = features<double>(rI, pI, enI, rII, pII, enII, gas.g, f);
     for (int k = 0; k < IN; k++) feats[k * n + i] = f[k];
-    // pattern test of kernel.cu:200-231; two shocks when Udf > Uud
-    const double a = sqrt(rI * (ph.g2 * pII + ph.g1 * pI) / 2.0);
-    const double Uud = (pII - pI) / a;
-    const bool two_shock = (enI - enII) > Uud;
+    // two shocks when the closing speed exceeds f_I(p_II) (kernel.cu:200-231)
+    const double du = enI - enII;
+    const bool two_shock = du > (pII - pI) / sqrt(0.5 * rI * (gas.g2 * pII + gas.g1 * pI));
     if (mask) mask[i] = two_shock ? 1 : 0;
     if (target) {
       float t = 0.f;
       if (two_shock) {
         int it;
         bool capped = false;
-        const double p = newton<double>(ph, enI, pI, rI, enII, pII, rII, false, 0.0, 10e-10, it, capped);
+        const double zI = rI * sqrt(gas.g * pI / rI), zII = rII * sqrt(gas.g * pII / rII);
+        const double p =
+            d64::two_shock_root(gas, du, pI, rI, zI, pII, rII, zII, -1.0, false, it, capped);
         t = (float)(log(p) - (double)lp);
       }
       target[i] = t;
@@ -306,17 +308,11 @@ extern "C" int ast_mlp_features(size_t n, const double* in, double n1, double n2
   if (n == 0) return AST_OK;
   if (!in || !feats) return ast_fail(AST_ERR_ARG, "ast_mlp_features: null pointer");
   if (!(gamma > 1.0)) return ast_fail(AST_ERR_ARG, "ast_mlp_features: gamma must be > 1");
-  Phys<double> ph;
-  ph.ga = gamma;
-  ph.g1 = gamma - 1.0;
-  ph.g2 = gamma + 1.0;
-  ph.gp = (ph.g2 / gamma) / 2.0;
-  ph.gm = (ph.g1 / gamma) / 2.0;
-  ph.krit = 1.0;
+  const d64::GasD gas = d64::make_gas(gamma);
   size_t blocks = (n + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(features_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n,
-                     in, n1, n2, ph, feats, target, mask);
+                     in, n1, n2, gas, feats, target, mask);
   AST_LAUNCH_CHECK("ast_mlp_features launch");
   return AST_OK;
 }

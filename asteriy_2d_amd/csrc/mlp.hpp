@@ -15,7 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "ref_flux.hpp"
+#include "dev.hpp"
 
 namespace ast {
 namespace mlp {

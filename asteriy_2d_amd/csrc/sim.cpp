@@ -29,6 +29,9 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
                       ast_stats_t* stats, int krows, const ast::StepTime* tm, void* stream);
 int ast_funk_time_fold(double* T, int last, int nslots, void* stream);
+int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, double* s2,
+                      double* u2, double* T, const double* T_do, int method, ast_stats_t* stats,
+                      int rows, void* stream);
 extern "C" int ast_get_tile_rows(void);
 
 #define CK(call)                                              \
@@ -41,7 +44,8 @@ struct ast_sim {
   ast_grid_t g;
   int precision;  // 0: FP64 planes, 1: FP32 planes
   int method;
-  int krows;
+  int krows;   // FP32 strip tiling (rows per strip, or -S strips per band)
+  int rows64;  // FP64 rows per strip (0: the kernel's default)
   size_t cells;
   void* plane[4];  // s0, u0, s1, u1
   int cur;         // 0: state in (s0,u0); 1: in (s1,u1)
@@ -87,8 +91,9 @@ static int launch_steps(ast_sim_t* S, int from, int n) {
     void* u2 = S->plane[2 * (1 - f) + 1];
     int rc;
     if (S->precision == 0) {
-      rc = ast_step_f64(&S->g, (const double*)s, (const double*)u, (double*)s2, (double*)u2, T,
-                        T + 1, T + 2, S->method, S->collect ? S->stats : nullptr, S->stream);
+      rc = ast_step_f64_rows(&S->g, (const double*)s, (const double*)u, (double*)s2,
+                             (double*)u2, T, T + 1, S->method, S->collect ? S->stats : nullptr,
+                             S->rows64, S->stream);
       if (rc == AST_OK) rc = ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
     } else {
       const ast::StepTime tm = step_time(S, j);
@@ -279,10 +284,13 @@ extern "C" int ast_sim_set_mlp(ast_sim_t* S, const ast_mlp_t* mlp) {
 // Strip tiling of the FP32 step (its tile shape; see the autotuner): rows
 // per wave strip, or -S for S strips of (nearly) equal height per band.
 extern "C" int ast_sim_set_tile_rows(ast_sim_t* S, int krows) {
-  if (!S || krows == 0 || krows > 1024 || (krows < 0 && -(long long)krows > S->g.M))
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_set_tile_rows: null sim");
+  if (S->precision == 0 ? (krows < 1 || krows > 4096)
+                        : (krows == 0 || krows > 1024 || (krows < 0 && -(long long)krows > S->g.M)))
     return ast_fail(AST_ERR_ARG, "ast_sim_set_tile_rows: %d", krows);
   CK(hipStreamSynchronize(S->stream));
-  S->krows = krows;
+  if (S->precision == 0) S->rows64 = krows;
+  else S->krows = krows;
   drop_graph(S);
   return AST_OK;
 }
@@ -301,8 +309,14 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_autotune: null sim");
   // default: strips of 4-16 rows, and the tilings that give every SIMD 2 to
   // 5 waves of (nearly) equal height, S strips per band with S * bands
-  // waves (a wave strip has 60 output columns; a CU has 4 SIMDs)
+  // waves (a wave strip has 60 output columns; a CU has 4 SIMDs); the FP64
+  // step takes strips of 4 to 32 rows
   int dflt[8] = {4, 8, 12, 16};
+  static const int dflt64[6] = {4, 6, 8, 12, 16, 32};
+  if ((!candidates || ncand <= 0) && S->precision == 0) {
+    candidates = dflt64;
+    ncand = 6;
+  }
   if (!candidates || ncand <= 0) {
     int nd = 4, dev = 0, cus = 0;
     CK(hipGetDevice(&dev));
@@ -327,7 +341,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   void* up = S->plane[2 * from + 1];
   void* s2 = S->plane[2 * (1 - from)];
   void* u2 = S->plane[2 * (1 - from) + 1];
-  int best = S->krows;
+  int best = S->precision == 1 ? S->krows : S->rows64;
   float best_t = 1e30f;
   // the first step of a run: dt from T_do, the learned mode from its word
   const ast::StepTime tm = step_time(S, 0);
@@ -336,19 +350,20 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
                ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                    (float*)u2, S->scal, S->scal + 1, S->method, S->mlp, nullptr,
                                    kr, &tm, S->stream)
-               : ast_step_f64(&S->g, (const double*)sp, (const double*)up, (double*)s2,
-                              (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method, nullptr,
-                              S->stream);
+               : ast_step_f64_rows(&S->g, (const double*)sp, (const double*)up, (double*)s2,
+                                   (double*)u2, S->scal, S->scal + 1, S->method, nullptr, kr,
+                                   S->stream);
   };
   auto valid = [&](int kr) {
-    return S->precision != 1 || !(kr == 0 || kr > 1024 || (kr < 0 && -kr > S->g.M));
+    if (S->precision == 0) return kr >= 1 && kr <= 4096;
+    return !(kr == 0 || kr > 1024 || (kr < 0 && -kr > S->g.M));
   };
-  const int n = S->precision == 1 ? ncand : 1;
+  const int n = ncand;
   // warm-up: the clocks of an idle device ramp over the first tens of
   // milliseconds of work, which would slow the first candidates; untimed
   // launches of the first valid candidate until 20 ms have passed
   for (int c = 0; c < n; c++) {
-    const int kr = S->precision == 1 ? candidates[c] : 0;
+    const int kr = candidates[c];
     if (!valid(kr)) continue;
     float ms = 0.f;
     CK(hipEventRecord(e0, S->stream));
@@ -364,7 +379,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
     break;
   }
   for (int c = 0; c < n; c++) {
-    const int kr = S->precision == 1 ? candidates[c] : 0;
+    const int kr = candidates[c];
     if (!valid(kr)) continue;
     for (int r = -2; r < reps; r++) {  // two untimed launches first
       if (r == 0) CK(hipEventRecord(e0, S->stream));
@@ -390,7 +405,11 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
     S->krows = best;
     drop_graph(S);
   }
-  if (best_rows) *best_rows = S->precision == 1 ? best : 0;
+  if (S->precision == 0 && best != S->rows64) {
+    S->rows64 = best;
+    drop_graph(S);
+  }
+  if (best_rows) *best_rows = best;
   if (best_us) *best_us = best_t;
   return AST_OK;
 }

@@ -195,7 +195,7 @@ def _run_sim(grid, method, precision, s, u, steps, mlp=None, autotune=False, sta
     if mlp is not None:
         sim.set_mlp(mlp)
     rows = None
-    if autotune and precision == "f32":
+    if autotune or precision == "f64":  # the FP64 step's strip height is tuned too
         rows, _ = sim.autotune()
     _prime(sim, s, u)
     ms = _timed(sim, steps, sim.torch_stream())

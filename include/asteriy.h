@@ -143,9 +143,11 @@ int ast_flux_f32(int method, size_t n, const float* in, float n1, float n2,
  * kernel.cu:2530): a deterministic min, not the reference's racy store
  * (kernel.cu:3013-3017).  TT is accepted for signature parity and unused,
  * as in the reference's live body.
- * ast_step_f64 is the FP64, cell-by-cell restatement of the reference step
- * (each cell evaluates its own four faces, in the reference's operation
- * order).  ast_step_f32 is the fast path: FP32 state, each face evaluated
+ * ast_step_f64 is the step in IEEE double: each face solved once and shared
+ * by its two cells, slopes from neighbour differences; it agrees with the
+ * reference's step (which solves each face from both cells with
+ * coordinate-difference slopes) to round-off, within 1e-12 relative per
+ * step.  ast_step_f32 is the fast path: FP32 state, each face evaluated
  * once and shared by its two cells, FP64 kept for the dt min and (exact
  * solver) the p* solve near vacuum; `mlp` (may be NULL) seeds the exact
  * solver's Newton iteration with the learned guess.  The fast path needs
@@ -253,7 +255,7 @@ int ast_mlp_get_params(const ast_mlp_t* mlp, float* host_params, void* stream);
  * ping-pong by swapping arguments, a device sync after every launch) with a
  * native object: it owns the two pairs of state planes and the device time
  * scalars (T, T_do, TT, i; initialised to 1e7, 1e-9, 0, 0 as in
- * kernel.cu:3321-3324) and replays two steps at a time from a captured HIP
+ * kernel.cu:3321-3324) and replays runs of 20 steps from a captured HIP
  * graph on its own stream.  precision 0 = FP64 (ast_step_f64), 1 = FP32
  * (ast_step_f32).  Host state crosses in the reference layout (N*M double
  * pairs s = {rho,p}, u = {vx,vy}). */
@@ -278,20 +280,21 @@ int ast_sim_collect_stats(ast_sim_t* sim, int on);
  * a device word, and only one of them runs -- the learned one in the steps
  * that follow a step with strong two-shock faces. */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
-/* Strip tiling of the FP32 step: rows > 0 rows per wave strip, rows = -S
+/* Strip tiling of the step.  FP32: rows > 0 rows per wave strip, rows = -S
  * S strips of (nearly) equal height per band of 60 columns; strips of 4 to
- * 64 rows inside the grid take the fast strip path. */
+ * 64 rows inside the grid take the fast strip path.  FP64: rows > 0 rows
+ * per wave strip (default 6). */
 int ast_sim_set_tile_rows(ast_sim_t* sim, int rows);
-/* Times the FP32 step for each candidate tiling (NULL = a default list:
- * 4, 8, 12, 16 rows and the tilings -S that give every SIMD 2, 3, 4 or 5
- * equal waves), keeps the fastest; reports it (may be NULL).  State and
- * time are left unchanged. */
+/* Times the step for each candidate tiling (NULL = a default list: FP32 4,
+ * 8, 12, 16 rows and the tilings -S that give every SIMD 2, 3, 4 or 5 equal
+ * waves; FP64 4, 6, 8, 12, 16, 32 rows), keeps the fastest; reports it (may
+ * be NULL).  State and time are left unchanged. */
 int ast_sim_autotune(ast_sim_t* sim, const int* candidates, int ncand, int reps,
                      int* best_rows, float* best_us);
 /* The simulation's stream (hipStream_t) for event timing. */
 void* ast_sim_stream(ast_sim_t* sim);
 /* Advances `steps` steps asynchronously (use_graph: replay the captured
- * two-step graph). */
+ * graph of 20 steps; a remainder is launched directly). */
 int ast_sim_run(ast_sim_t* sim, int steps, int use_graph);
 int ast_sim_sync(ast_sim_t* sim);
 /* The reference's binary state file (kernel.cu:3414-3436 read,

@@ -1,0 +1,124 @@
+"""Host-side test cases with closed-form answers, shared by the oracle tests
+(CPU) and the GPU tests."""
+import math
+
+import numpy as np
+
+from asteriy_2d_amd import grid as G
+
+
+def floor_case(T_do, w=0.6, a=2.0, gamma=5.0 / 3.0, geometry=G.PLANAR, first_order=True):
+    """A 5x2 planar grid whose middle column (n = 2) sits at rest between
+    columns moving away from it at speed a, every cell with rho = p = 1 and
+    tangential velocity w, all faces first order (HLL on the cell states,
+    kernel.cu:2812, 2856).  The rows are equal and the y ghosts copy the edge
+    rows, so the y fluxes cancel.  Returns (grid, s, u, expected middle cell
+    (rho', p', vx', vy') by the closed form below).  first_order=False
+    reconstructs every face instead (no closed form then: the expected
+    values are those of the first-order case, for orientation only)."""
+    ho = (0.0, 0.0) if first_order else (-math.inf, math.inf)
+    g = G.Grid.from_extent(5, 2, 0.0, 0.4, 0.2, gamma=gamma, krit=0.3, geometry=geometry,
+                           bc=(G.BC_OUTFLOW,) * 4, ho_lo=ho[0], ho_hi=ho[1])
+    s = np.ones((2, 5, 2))
+    u = np.zeros((2, 5, 2))
+    u[:, :2, 0], u[:, 3:, 0] = -a, a
+    u[..., 1] = w
+    # HLL with Davis speeds at the east face of the middle cell: SL = -c,
+    # SR = a + c; its mass flux c a/(a + 2c), tangential momentum c a w/(a + 2c),
+    # energy c ((E_R + p) a - (a + c) a^2/2)/(a + 2c) with E_R = p/(g-1) +
+    # (a^2 + w^2)/2 (the w^2 parts of E_R - E_L cancel); the west face is its
+    # mirror image (mass and energy leave through it as well, the normal
+    # momentum flux is even and cancels)
+    c = math.sqrt(gamma)
+    E0 = 1.0 / (gamma - 1) + 0.5 * w * w
+    ER = 1.0 / (gamma - 1) + 0.5 * (a * a + w * w)
+    k = T_do / g.dx
+    dr = -2 * k * c * a / (a + 2 * c)
+    dE = -2 * k * c * ((ER + 1.0) * a - (a + c) * a * a / 2) / (a + 2 * c)
+    dmy = w * dr
+    r2 = 1.0 + dr
+    if r2 <= 0:
+        r2 = 1.0                    # kernel.cu:3060-3064: keep the old density
+    vy = (w + dmy) / r2
+    p2 = (E0 + dE - 0.5 * r2 * vy * vy) * (gamma - 1)
+    if p2 <= 0:
+        p2 = 1e-6                   # kernel.cu:3070-3073
+    return g, s, u, (r2, p2, 0.0, vy)
+
+
+# T_do of the three regimes of floor_case: no floor, the density floor alone,
+# both floors
+FLOOR_REGIMES = [(0.005, False, False), (0.1, True, False), (0.3, True, True)]
+
+
+def uniform_axisymmetric(T_do=0.01, rho=0.8, p=1.3, ux=0.45, uy=-0.7):
+    """A uniform state with v != 0 on an axisymmetric grid with zero-gradient
+    ghosts on every side: every face carries the same flux, so one step
+    changes each cell only by the r-source -(T_do/y) v {rho, rho u, rho v,
+    gamma p/(g-1) + rho|u|^2/2} at the cell-centre radius y
+    (KorolkovSolver.cu:13-32).  Worked through: rho' = rho (1 - f), u' = u,
+    v' = v, p' = p (1 - gamma f) with f = T_do v / y.  Returns (grid, s, u,
+    expected s2, expected u2)."""
+    g = G.Grid.from_extent(12, 9, -1.0, 1.0, 0.9, gamma=5.0 / 3.0, krit=0.3,
+                           geometry=G.AXISYMMETRIC, bc=(G.BC_OUTFLOW,) * 4)
+    s = np.empty((g.M, g.N, 2))
+    s[..., 0], s[..., 1] = rho, p
+    u = np.empty_like(s)
+    u[..., 0], u[..., 1] = ux, uy
+    y = g.y(np.arange(g.M))[:, None] * np.ones((1, g.N))
+    f = T_do * uy / y
+    s2 = np.stack([rho * (1 - f), p * (1 - g.gamma * f)], -1)
+    u2 = u.copy()
+    return g, s, u, s2, u2
+
+
+# ---- comparison of an FP64 result with the oracle ---------------------------
+#
+# The FP64 GPU path and the oracle compute the same step with different (but
+# algebraically equal) operation orders: the GPU solves each face once, the
+# oracle (like the reference) twice with coordinate-difference slopes.  They
+# agree to round-off; the bar is 1e-12 relative per step, measured per cell
+# against the natural scale of each quantity: rho and p against their own
+# value, each velocity component against the local signal speed |u| + c.
+
+PARITY_REL = 1e-12
+
+
+def step_mismatch(s2, u2, rs2, ru2, gamma, rel=PARITY_REL):
+    """Number of cells (and the worst relative difference) where an FP64 step
+    result (s2, u2) differs from the oracle's (rs2, ru2) by more than rel."""
+    s2, u2, rs2, ru2 = (np.asarray(a, dtype=np.float64) for a in (s2, u2, rs2, ru2))
+    c = np.sqrt(gamma * np.abs(rs2[..., 1]) / np.abs(rs2[..., 0]))
+    vs = np.sqrt(ru2[..., 0] ** 2 + ru2[..., 1] ** 2) + c
+    d = np.stack([np.abs(s2[..., 0] - rs2[..., 0]) / np.abs(rs2[..., 0]),
+                  np.abs(s2[..., 1] - rs2[..., 1]) / np.abs(rs2[..., 1]),
+                  np.abs(u2[..., 0] - ru2[..., 0]) / vs,
+                  np.abs(u2[..., 1] - ru2[..., 1]) / vs], -1)
+    d = np.where(np.isnan(d), np.inf, d)
+    return int((d > rel).any(-1).sum()), float(d.max())
+
+
+def flux_scales(F, gamma):
+    """Natural scale of each output plane of a face batch F (8, n): the mass
+    flux rho (|u| + c), the energy flux (E + p)(|u| + c), the momentum fluxes
+    rho (|u| + c)^2 + p, with the larger density, pressure, speed and sound
+    speed of the two sides."""
+    r = np.maximum(F[0], F[4])
+    p = np.maximum(F[1], F[5])
+    c = np.sqrt(gamma * np.maximum(F[1] / F[0], F[5] / F[4]))
+    v = np.abs(F[[2, 3, 6, 7]]).max(0) + c
+    E = p / (gamma - 1) + 0.5 * r * v * v
+    mom = r * v * v + p
+    return np.stack([r * v, (E + p) * v, mom, mom])
+
+
+def flux_mismatch(out, ref, F, gamma, rel=PARITY_REL):
+    """(count, worst) of flux values (4 planes) differing from the oracle by
+    more than rel of their natural scale, plus dt (plane 4) relative."""
+    sc = flux_scales(F, gamma)
+    d = np.abs(out[:4] - ref[:4]) / sc
+    both_nan = np.isnan(out[:4]) & np.isnan(ref[:4])
+    d = np.where(both_nan, 0.0, np.where(np.isnan(d), np.inf, d))
+    dt = np.abs(out[4] - ref[4]) / np.abs(ref[4])
+    dt = np.where((out[4] == ref[4]), 0.0, np.where(np.isnan(dt), np.inf, dt))
+    return int((d > rel).sum() + (dt > rel).sum()), float(max(d.max(), dt.max()))

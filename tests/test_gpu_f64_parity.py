@@ -1,9 +1,11 @@
 """GPU parity of the FP64 path against the CPU oracle (through the C-ABI).
 
-The FP64 kernels restate the reference step with its operation order and
-no FMA contraction, so HLL and HLLC faces/steps must agree bit for bit; the
-exact solver calls pow (ROCm's ocml on the GPU, glibc on the CPU, each
-within 1 ulp), so it is held to 1e-12 relative.
+The FP64 step solves each face once and reconstructs from neighbour
+differences; the oracle, like the reference, solves every face from both of
+its cells with coordinate-difference slopes.  The two agree to round-off: the
+bar is 1e-12 relative per step (tests/cases.py: rho and p against their
+value, velocities against |u| + c; fluxes against their natural scale), and
+the exact solver's wave patterns and iteration counts are equal.
 """
 import numpy as np
 import pytest
@@ -11,23 +13,16 @@ import pytest
 from asteriy_2d_amd import grid as G
 from asteriy_2d_amd import facegen, problems
 
+from cases import PARITY_REL, flux_mismatch, step_mismatch
+
 pytestmark = pytest.mark.gpu
 
-REL = 1e-12
+REL = PARITY_REL
 
 
 def _t(a, dev):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-
-
-def _close(a, b, rel=REL, scale=None):
-    a, b = np.asarray(a), np.asarray(b)
-    if scale is None:
-        scale = np.abs(b)
-    bad = ~(np.abs(a - b) <= rel * scale)
-    bad &= ~(np.isnan(a) & np.isnan(b))
-    return int(bad.sum())
 
 
 @pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
@@ -40,14 +35,9 @@ def test_flux_f64_matches_oracle(dev, ora, method):
     out = solver.flux_batch(method, _t(faces, dev), 1.0, 0.0, 0.25).cpu().numpy()
     torch.cuda.synchronize()
     ref = ora.flux_batch(method, faces, 1.0, 0.0, 0.25)
-    if method in (G.HLL, G.HLLC, G.HLLC_A):
-        same = (out == ref) | (np.isnan(out) & np.isnan(ref))
-        assert same.all(), f"{(~same).sum()} values differ bitwise"
-    else:
-        # per-plane scale: the flux magnitude of the row (mass, energy, momentum)
-        scale = np.maximum(np.abs(ref), 1e-300)
-        for k in range(5):
-            assert _close(out[k], ref[k], REL, scale[k]) == 0, f"plane {k}"
+    bad, worst = flux_mismatch(out, ref, faces, 5.0 / 3.0)
+    print(f"  worst relative difference {worst:.2e}")
+    assert bad == 0, (bad, worst)
 
 
 @pytest.mark.parametrize("normal", [(1.0, 0.0), (0.0, -1.0), (-1.0, 0.0), (0.0, 1.0)])
@@ -57,7 +47,8 @@ def test_flux_f64_normals(dev, ora, normal):
     for method in (G.HLL, G.HLLC, G.EXACT, G.HLLC_A):
         out = solver.flux_batch(method, _t(faces, dev), *normal, 0.5).cpu().numpy()
         ref = ora.flux_batch(method, faces, *normal, 0.5)
-        assert _close(out, ref, REL, np.maximum(np.abs(ref), 1e-300)) == 0
+        bad, worst = flux_mismatch(out, ref, faces, 5.0 / 3.0)
+        assert bad == 0, (method, bad, worst)
 
 
 def _gpu_step(grid, s, u, T, T_do, method, dev, stats=None):
@@ -99,14 +90,10 @@ def test_step_f64_matches_oracle(dev, ora, name, grid, method):
     T_do = 0.01
     s2, u2, T = _gpu_step(grid, s, u, 1e7, T_do, method, dev)
     rs2, ru2, rT = ora.step(grid, s, u, 1e7, T_do, method)
-    if method == G.EXACT:
-        assert _close(s2, rs2) == 0
-        assert _close(u2, ru2, REL, np.abs(ru2) + 1.0) == 0
-        assert abs(T - rT) <= REL * rT
-    else:
-        assert np.array_equal(s2, rs2), f"s2: {(s2 != rs2).sum()} differ"
-        assert np.array_equal(u2, ru2), f"u2: {(u2 != ru2).sum()} differ"
-        assert T == rT
+    bad, worst = step_mismatch(s2, u2, rs2, ru2, grid.gamma)
+    print(f"  worst relative difference {worst:.2e}")
+    assert bad == 0, (bad, worst)
+    assert abs(T - rT) <= REL * rT
 
 
 def test_reference_live_configuration(dev, ora):
@@ -118,7 +105,8 @@ def test_reference_live_configuration(dev, ora):
     for method in (G.HLLC,):
         s2, u2, T = _gpu_step(g, s, u, 1e7, 1e-4, method, dev)
         rs2, ru2, rT = ora.step(g, s, u, 1e7, 1e-4, method)
-        assert np.array_equal(s2, rs2) and np.array_equal(u2, ru2) and T == rT
+        assert step_mismatch(s2, u2, rs2, ru2, g.gamma)[0] == 0
+        assert abs(T - rT) <= REL * rT
 
 
 def test_sod_200_steps_every_step(dev, ora):
@@ -140,9 +128,9 @@ def test_sod_200_steps_every_step(dev, ora):
         solver.Kernel_TVD(g, S, U, S2, U2, T, Td, TT, p.method)
         T_after = float(T.item())
         rs2, ru2, rT = ora.step(g, s_in, u_in, 1e7, T_do, p.method)
-        assert np.array_equal(S2.cpu().numpy(), rs2), f"step {k}"
-        assert np.array_equal(U2.cpu().numpy(), ru2), f"step {k}"
-        assert T_after == rT, f"step {k}"
+        bad, worst = step_mismatch(S2.cpu().numpy(), U2.cpu().numpy(), rs2, ru2, g.gamma)
+        assert bad == 0, (k, bad, worst)
+        assert abs(T_after - rT) <= REL * rT, f"step {k}"
         solver.funk_time(T, Td, TT, it)
         S, S2, U, U2 = S2, S, U2, U
     assert int(it.item()) == p.steps

@@ -21,6 +21,8 @@ import pytest
 from asteriy_2d_amd import grid as G
 from asteriy_2d_amd import facegen, problems
 
+from cases import FLOOR_REGIMES, floor_case, uniform_axisymmetric
+
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
@@ -449,75 +451,21 @@ def test_vacuum_branch_matches_toro_vacuum_solution(ora):
 
 @pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
 def test_axisymmetric_source_closed_form(ora, method):
-    """A uniform state (rho, p, u, v) with zero-gradient ghosts on every side
-    has equal fluxes through every face, so one axisymmetric step changes each
-    cell only by the r-source -(T_do/y) v {rho, rho u, rho v, gamma p/(g-1) +
-    rho|u|^2/2} at the cell-centre radius y.  Worked through: rho' = rho (1 -
-    T_do v/y), u' = u, v' = v, p' = p (1 - gamma T_do v/y).  A wrong sign, a
-    dropped factor gamma, the kinetic term left out of the energy source or a
-    node radius in place of the centre radius all fail."""
-    g = G.Grid.from_extent(12, 9, -1.0, 1.0, 0.9, gamma=5.0 / 3.0, krit=0.3,
-                           geometry=G.AXISYMMETRIC, bc=(G.BC_OUTFLOW,) * 4)
-    rho, p, ux, uy = 0.8, 1.3, 0.45, -0.7
-    s = np.empty((g.M, g.N, 2))
-    s[..., 0], s[..., 1] = rho, p
-    u = np.empty_like(s)
-    u[..., 0], u[..., 1] = ux, uy
+    """uniform_axisymmetric: one step of a uniform state with v != 0 changes
+    each cell only by the r-source.  A wrong sign, a dropped factor gamma, the
+    kinetic term left out of the energy source or a node radius in place of
+    the centre radius all fail."""
     T_do = 0.01
+    g, s, u, ws, wu = uniform_axisymmetric(T_do)
     s2, u2, _ = ora.step(g, s, u, 1e7, T_do, method)
-    y = g.y(np.arange(g.M))[:, None] * np.ones((1, g.N))
     assert abs(g.y(0) - 0.05) < 1e-15  # centre radius y_min + m dy, y_min = dy/2
-    f = T_do * uy / y
-    assert np.abs(s2[..., 0] - rho * (1 - f)).max() <= 1e-14
-    assert np.abs(s2[..., 1] - p * (1 - g.gamma * f)).max() <= 1e-14
-    assert np.abs(u2[..., 0] - ux).max() <= 1e-14
-    assert np.abs(u2[..., 1] - uy).max() <= 1e-14
+    assert np.abs(s2 - ws).max() <= 1e-14
+    assert np.abs(u2 - wu).max() <= 1e-14
     # the source really acted (v != 0): the change is not round-off
-    assert np.abs(s2[..., 0] - rho).min() > 1e-4
+    assert np.abs(s2[..., 0] - s[..., 0]).min() > 1e-4
 
 
 # ---- the floors of the update (kernel.cu:3060-3073) ------------------------------
-
-def floor_case(T_do, w=0.6, a=2.0, gamma=5.0 / 3.0, geometry=G.PLANAR):
-    """A 5x2 planar grid whose middle column (n = 2) sits at rest between
-    columns moving away from it at speed a, every cell with rho = p = 1 and
-    tangential velocity w, all faces first order (HLL on the cell states,
-    kernel.cu:2812, 2856).  The rows are equal and the y ghosts copy the edge
-    rows, so the y fluxes cancel.  Returns (grid, s, u, expected middle cell
-    (rho', p', vx', vy') by the closed form below)."""
-    g = G.Grid.from_extent(5, 2, 0.0, 0.4, 0.2, gamma=gamma, krit=0.3, geometry=geometry,
-                           bc=(G.BC_OUTFLOW,) * 4, ho_lo=0.0, ho_hi=0.0)
-    s = np.ones((2, 5, 2))
-    u = np.zeros((2, 5, 2))
-    u[:, :2, 0], u[:, 3:, 0] = -a, a
-    u[..., 1] = w
-    # HLL with Davis speeds at the east face of the middle cell: SL = -c,
-    # SR = a + c; its mass flux c a/(a + 2c), tangential momentum c a w/(a + 2c),
-    # energy c ((E_R + p) a - (a + c) a^2/2)/(a + 2c) with E_R = p/(g-1) +
-    # (a^2 + w^2)/2 (the w^2 parts of E_R - E_L cancel); the west face is its
-    # mirror image (mass and energy leave through it as well, the normal
-    # momentum flux is even and cancels)
-    c = math.sqrt(gamma)
-    E0 = 1.0 / (gamma - 1) + 0.5 * w * w
-    ER = 1.0 / (gamma - 1) + 0.5 * (a * a + w * w)
-    k = T_do / g.dx
-    dr = -2 * k * c * a / (a + 2 * c)
-    dE = -2 * k * c * ((ER + 1.0) * a - (a + c) * a * a / 2) / (a + 2 * c)
-    dmy = w * dr
-    r2 = 1.0 + dr
-    if r2 <= 0:
-        r2 = 1.0                    # kernel.cu:3060-3064: keep the old density
-    vy = (w + dmy) / r2
-    p2 = (E0 + dE - 0.5 * r2 * vy * vy) * (gamma - 1)
-    if p2 <= 0:
-        p2 = 1e-6                   # kernel.cu:3070-3073
-    return g, s, u, (r2, p2, 0.0, vy)
-
-
-# T_do of the three regimes of floor_case: no floor, the density floor alone,
-# both floors
-FLOOR_REGIMES = [(0.005, False, False), (0.1, True, False), (0.3, True, True)]
-
 
 @pytest.mark.parametrize("T_do,rho_floor,p_floor", FLOOR_REGIMES)
 def test_update_floors_closed_form(ora, T_do, rho_floor, p_floor):
