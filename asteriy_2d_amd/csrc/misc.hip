@@ -1,5 +1,7 @@
 // misc.hip -- small kernels around the step: the time advance (funk_time),
 // FP64 <-> FP32 state conversion, and compensated FP64 conservation sums.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace ast {
@@ -71,6 +73,13 @@ struct KSum {
 };
 
 constexpr int SUM_BLOCKS = 256, SUM_NT = 256;
+
+// Scratch of ast_conservation_sums_noalloc: the per-workgroup partial sums
+// of pass 1 (4 doubles per workgroup, 8 KiB).  Each device holds its own
+// copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (scratch_on_device).
+constexpr int kScratchDoubles = 4 * SUM_BLOCKS;
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 // Pass 1: block b sums rows [b*M/B, (b+1)*M/B) in a fixed order and writes
 // its (sum, compensation) pairs for mass and energy to work[4*b..4*b+3].
@@ -164,6 +173,53 @@ extern "C" int ast_convert_f32_to_f64(size_t n, const float* src, double* dst, v
 }
 
 extern "C" size_t ast_conservation_work_doubles(void) { return 2 + 4 * ast::SUM_BLOCKS; }
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+static int scratch_on_device(double** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = 0;
+  AST_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return ast_fail(AST_ERR_SIZE, "ast_conservation_sums_noalloc: device %d beyond %d", dev,
+                    kMaxDevices);
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(ast::scratch));
+    if (e != hipSuccess || p == nullptr)
+      return ast_hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                          "ast_conservation_sums_noalloc: hipGetSymbolAddress(scratch)");
+    addr[dev] = static_cast<double*>(p);
+  }
+  *out = addr[dev];
+  return AST_OK;
+}
+
+extern "C" int ast_conservation_sums_noalloc(const ast_grid_t* g, const void* s, const void* u,
+                                             int fp32, double* out, void* stream) {
+  int rc = ast_grid_check(g);
+  if (rc != AST_OK) return rc;
+  if (!s || !u || !out)
+    return ast_fail(AST_ERR_ARG, "ast_conservation_sums_noalloc: null pointer");
+  // pass 1 writes 4 partials per workgroup, whatever the grid size
+  static_assert(4 * ast::SUM_BLOCKS <= ast::kScratchDoubles, "scratch holds every partial");
+  double* work = nullptr;
+  if ((rc = scratch_on_device(&work)) != AST_OK) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (fp32)
+    hipLaunchKernelGGL(ast::sums_pass1<float2>, dim3(ast::SUM_BLOCKS), dim3(ast::SUM_NT), 0, st,
+                       *g, (const float2*)s, (const float2*)u, work);
+  else
+    hipLaunchKernelGGL(ast::sums_pass1<double2>, dim3(ast::SUM_BLOCKS), dim3(ast::SUM_NT), 0, st,
+                       *g, (const double2*)s, (const double2*)u, work);
+  AST_LAUNCH_CHECK("ast_conservation_sums_noalloc pass 1");
+  hipLaunchKernelGGL(ast::sums_pass2, dim3(1), dim3(64), 0, st, work, out);
+  AST_LAUNCH_CHECK("ast_conservation_sums_noalloc pass 2");
+  return AST_OK;
+}
 
 extern "C" int ast_conservation_sums(const ast_grid_t* g, const void* s, const void* u, int fp32,
                                      double* out, void* stream) {

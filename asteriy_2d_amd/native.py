@@ -33,6 +33,7 @@ SIGNATURES = {
     "ast_funk_time": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ast_conservation_work_doubles": (_c_size, []),
     "ast_conservation_sums": (_c_int, [_gp, _vp, _vp, _c_int, _vp, _vp]),
+    "ast_conservation_sums_noalloc": (_c_int, [_gp, _vp, _vp, _c_int, _vp, _vp]),
     "ast_convert_f64_to_f32": (_c_int, [_c_size, _vp, _vp, _vp]),
     "ast_convert_f32_to_f64": (_c_int, [_c_size, _vp, _vp, _vp]),
     "ast_flux_f32_info": (_c_int, [_c_int, _c_size, _vp, _c_float, _c_float, _c_float, _c_float,

@@ -178,6 +178,19 @@ int ast_funk_time(double* T, double* T_do, double* TT, int* i, void* stream);
 int ast_conservation_sums(const ast_grid_t* g, const void* s, const void* u,
                           int fp32, double* out, void* stream);
 
+/* ast_conservation_sums without the workspace: the partial sums live in a
+ * fixed 8 KiB device array of the library (one per GPU, held until the
+ * process ends), so the call allocates nothing and needs only `out` (2
+ * doubles, device).  No counterpart in the reference: it is for callers that
+ * cannot allocate -- code under HIP graph capture, tight stepping or serving
+ * loops, harnesses that forbid hipMalloc.  Its scratch use does not grow
+ * with the grid (256 workgroups, 4 partials each), so every valid grid fits.
+ * Calls on one GPU share that scratch: calls that overlap in time on one
+ * GPU (other streams, other threads) race and must be ordered by the
+ * caller. */
+int ast_conservation_sums_noalloc(const ast_grid_t* g, const void* s, const void* u,
+                                  int fp32, double* out, void* stream);
+
 /* Converts between the FP64 and FP32 state planes (n pairs). */
 int ast_convert_f64_to_f32(size_t n, const double* src, float* dst, void* stream);
 int ast_convert_f32_to_f64(size_t n, const float* src, double* dst, void* stream);
