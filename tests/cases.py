@@ -78,8 +78,13 @@ def uniform_axisymmetric(T_do=0.01, rho=0.8, p=1.3, ux=0.45, uy=-0.7):
 # algebraically equal) operation orders: the GPU solves each face once, the
 # oracle (like the reference) twice with coordinate-difference slopes.  They
 # agree to round-off; the bar is 1e-12 relative per step, measured per cell
-# against the natural scale of each quantity: rho and p against their own
-# value, each velocity component against the local signal speed |u| + c.
+# against the natural scale of each quantity: rho against its value, each
+# velocity component against the local signal speed |u| + c, and p against
+# the energy it is recovered from, (gamma-1) E = p + (gamma-1) rho |u|^2/2.
+# (The reference forms p' = (E' - rho'|u'|^2/2)(gamma-1): at Mach M the two
+# terms cancel to 1 part in 1 + gamma(gamma-1)M^2/2, so its own rounding
+# moves p by that many ulps -- ~3000 at the Mach-70 edge of the wind bubble,
+# 7e-13 of p; the GPU step forms p' without that cancellation.)
 
 PARITY_REL = 1e-12
 
@@ -89,9 +94,11 @@ def step_mismatch(s2, u2, rs2, ru2, gamma, rel=PARITY_REL):
     result (s2, u2) differs from the oracle's (rs2, ru2) by more than rel."""
     s2, u2, rs2, ru2 = (np.asarray(a, dtype=np.float64) for a in (s2, u2, rs2, ru2))
     c = np.sqrt(gamma * np.abs(rs2[..., 1]) / np.abs(rs2[..., 0]))
-    vs = np.sqrt(ru2[..., 0] ** 2 + ru2[..., 1] ** 2) + c
+    v2 = ru2[..., 0] ** 2 + ru2[..., 1] ** 2
+    vs = np.sqrt(v2) + c
+    pe = np.abs(rs2[..., 1]) + (gamma - 1) * 0.5 * np.abs(rs2[..., 0]) * v2
     d = np.stack([np.abs(s2[..., 0] - rs2[..., 0]) / np.abs(rs2[..., 0]),
-                  np.abs(s2[..., 1] - rs2[..., 1]) / np.abs(rs2[..., 1]),
+                  np.abs(s2[..., 1] - rs2[..., 1]) / pe,
                   np.abs(u2[..., 0] - ru2[..., 0]) / vs,
                   np.abs(u2[..., 1] - ru2[..., 1]) / vs], -1)
     d = np.where(np.isnan(d), np.inf, d)

@@ -1,0 +1,228 @@
+"""GPU checks on the benchmark workloads themselves, against the CPU oracle.
+
+* C3, the wind / counter-flow interaction with FP64 HLLC: every GPU step
+  compared with the oracle step started from the same input (1e-12 per
+  step), 50 steps at 256x128 and 3 steps at the full 2048x1024.
+* C4, the exact solver in FP32 (FP64 islands), with and without the learned
+  newton start: 100 steps at 512x256 against 100 oracle steps, L1(rho) and
+  L1(p) <= 1e-5; at the full 4096x2048, 20 steps against the FP64 GPU step
+  (L1 <= 1e-5) and through properties (finite, positive, the lagged dt, the
+  conservation sums).
+* C5, the 64-instance sweep (seed 5): finite and positive everywhere, four
+  instances bit for bit equal to single-grid runs, the smallest one against
+  the oracle.
+* The FP32 step on the parity grids against the oracle step directly.
+"""
+import numpy as np
+import pytest
+
+from asteriy_2d_amd import grid as G
+from asteriy_2d_amd import problems
+
+from cases import PARITY_REL, step_mismatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _l1(a, b):
+    return float(np.abs(a - b).sum() / np.abs(b).sum())
+
+
+def _t(a, dev, dt=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    return t.to(dt) if dt is not None else t
+
+
+def _every_step_fp64(ora, dev, p, steps, method):
+    """GPU FP64 steps of problem p, each compared with the oracle step from the
+    same input; the reference protocol (step, funk_time) in between."""
+    import torch
+    from asteriy_2d_amd import solver
+    g = p.grid
+    S, U = _t(p.s, dev), _t(p.u, dev)
+    S2, U2 = torch.empty_like(S), torch.empty_like(U)
+    T = torch.tensor([1e7], dtype=torch.float64, device=dev)
+    Td = torch.tensor([1e-9], dtype=torch.float64, device=dev)
+    TT = torch.zeros(1, dtype=torch.float64, device=dev)
+    worst = 0.0
+    for k in range(steps):
+        s_in, u_in, T_do = S.cpu().numpy(), U.cpu().numpy(), float(Td.item())
+        solver.Kernel_TVD(g, S, U, S2, U2, T, Td, TT, method)
+        rs2, ru2, rT = ora.step(g, s_in, u_in, 1e7, T_do, method)
+        bad, w = step_mismatch(S2.cpu().numpy(), U2.cpu().numpy(), rs2, ru2, g.gamma)
+        assert bad == 0, (k, bad, w)
+        assert abs(float(T.item()) - rT) <= PARITY_REL * rT, k
+        worst = max(worst, w)
+        solver.funk_time(T, Td, TT)
+        S, S2, U, U2 = S2, S, U2, U
+    return worst
+
+
+def test_c3_wind_fp64_every_step(dev, ora):
+    p = problems.wind(256, 128)
+    print("C3 wind 256x128 FP64 HLLC, 50 steps", flush=True)
+    w = _every_step_fp64(ora, dev, p, 50, G.HLLC)
+    print(f"  worst per-step relative difference {w:.2e}")
+
+
+def test_c3_wind_fp64_full_size(dev, ora):
+    p = problems.wind(2048, 1024)
+    print("C3 wind 2048x1024 FP64 HLLC, 3 steps", flush=True)
+    w = _every_step_fp64(ora, dev, p, 3, G.HLLC)
+    print(f"  worst per-step relative difference {w:.2e}")
+
+
+def test_c3_wind_fp64_exact_every_step(dev, ora):
+    p = problems.wind(128, 64, method=G.EXACT)
+    print("wind 128x64 FP64 exact, 20 steps", flush=True)
+    _every_step_fp64(ora, dev, p, 20, G.EXACT)
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_c4_exact_fp32_vs_oracle(dev, ora, learned):
+    """100 steps of the FP32 exact step (FP64 islands; optionally the learned
+    newton start) against 100 oracle steps from the same FP32-rounded state:
+    L1 of rho and p <= 1e-5."""
+    from asteriy_2d_amd import learned as Lg
+    from asteriy_2d_amd.sim import Simulation
+    p = problems.wind(512, 256, method=G.EXACT)
+    s0 = p.s.astype(np.float32).astype(np.float64)
+    u0 = p.u.astype(np.float32).astype(np.float64)
+    print(f"C4 exact FP32 512x256 learned={learned}, 100 steps", flush=True)
+    with Simulation(p.grid, G.EXACT, "f32", s0, u0) as sim:
+        if learned:
+            sim.set_mlp(Lg.LearnedGuess.load())
+        sim.run(100)
+        s, u = sim.state()
+        t = sim.time()
+    ref = ora.Sim(p.grid, s0, u0, G.EXACT).advance(100)
+    e_r, e_p = _l1(s[..., 0], ref.s[..., 0]), _l1(s[..., 1], ref.s[..., 1])
+    print(f"  L1 rho {e_r:.2e}  L1 p {e_p:.2e}")
+    assert e_r <= 1e-5 and e_p <= 1e-5
+    assert t["i"] == 100 and abs(t["TT"] - ref.TT) <= 1e-5 * ref.TT
+
+
+def test_c4_full_size_properties(dev):
+    """4096x2048, FP32 exact with the learned start, 20 steps: finite and
+    positive; the first step's dt (the T_do the second step uses) equals the
+    FP64 step's dt to FP32 accuracy; L1 of rho and p against 20 FP64 GPU steps
+    <= 1e-5; the compensated conservation sums of the two runs agree to 1e-6
+    and equal numpy's FP64 sums of the same state to 1e-12."""
+    import torch
+    from asteriy_2d_amd import learned as Lg
+    from asteriy_2d_amd import solver
+    from asteriy_2d_amd.sim import Simulation
+    p = problems.wind(4096, 2048, method=G.EXACT)
+    g = p.grid
+    s0 = p.s.astype(np.float32).astype(np.float64)
+    u0 = p.u.astype(np.float32).astype(np.float64)
+    print("C4 exact FP32 + learned 4096x2048, 20 steps", flush=True)
+    with Simulation(g, G.EXACT, "f32", s0, u0) as a:
+        a.set_mlp(Lg.LearnedGuess.load())
+        a.run(1)
+        dt1 = a.time()["T_do"]
+        a.run(19)
+        s32, u32 = a.state()
+        t32 = a.time()
+    with Simulation(g, G.EXACT, "f64", s0, u0) as b:
+        b.run(1)
+        dt1_64 = b.time()["T_do"]
+        b.run(19)
+        s64, u64 = b.state()
+        t64 = b.time()
+    assert np.isfinite(s32).all() and np.isfinite(u32).all()
+    assert (s32[..., 0] > 0).all() and (s32[..., 1] > 0).all()
+    assert abs(dt1 - dt1_64) <= 1e-5 * dt1_64
+    assert t32["i"] == 20 and abs(t32["TT"] - t64["TT"]) <= 1e-5 * t64["TT"]
+    e_r, e_p = _l1(s32[..., 0], s64[..., 0]), _l1(s32[..., 1], s64[..., 1])
+    print(f"  L1 rho {e_r:.2e}  L1 p {e_p:.2e}")
+    assert e_r <= 1e-5 and e_p <= 1e-5
+    sums = []
+    for s, u in ((s32, u32), (s64, u64)):
+        out = solver.conservation_sums(g, _t(s, dev), _t(u, dev)).cpu().numpy()
+        w = g.dx * g.dy * g.y(np.arange(g.M))[:, None]
+        mass = float((s[..., 0] * w).sum())
+        energy = float(((s[..., 1] / (g.gamma - 1) + 0.5 * s[..., 0] * (u ** 2).sum(-1)) * w).sum())
+        assert abs(out[0] - mass) <= 1e-12 * mass and abs(out[1] - energy) <= 1e-12 * energy
+        sums.append(out)
+    assert np.all(np.abs(sums[0] - sums[1]) <= 1e-6 * np.abs(sums[1]))
+    torch.cuda.synchronize()
+
+
+def test_c5_sweep(dev, ora):
+    """The 64-instance sweep (seed 5), 40 steps: every instance finite and
+    positive; instances 0, 5, 17 and 42 equal single-grid FP32 runs bit for
+    bit; the first 256x128 instance within 1e-5 (L1 rho, p) of 40 oracle
+    steps from the same FP32-rounded state."""
+    from asteriy_2d_amd import batch as B
+    from asteriy_2d_amd.sim import Simulation
+    inst = B.sweep(64, seed=5)
+    steps = 40
+    print(f"C5 sweep: 64 instances, {steps} steps", flush=True)
+    with B.Batch([x[0] for x in inst], G.HLLC, [(x[1], x[2]) for x in inst]) as bt:
+        bt.autotune(reps=2)
+        bt.run(steps)
+        states = [bt.state(k) for k in range(64)]
+        times = [bt.time(k) for k in range(64)]
+    for k, (s, u) in enumerate(states):
+        assert np.isfinite(s).all() and np.isfinite(u).all(), k
+        assert (s[..., 0] > 0).all() and (s[..., 1] > 0).all(), k
+        assert times[k]["i"] == steps
+    for k in (0, 5, 17, 42):
+        g, s, u, _ = inst[k]
+        print(f"  instance {k}: {g.N}x{g.M}", flush=True)
+        with Simulation(g, G.HLLC, "f32", s, u) as sim:
+            sim.run(steps)
+            rs, ru = sim.state()
+        assert np.array_equal(states[k][0], rs) and np.array_equal(states[k][1], ru), k
+    k = next(i for i, x in enumerate(inst) if (x[0].N, x[0].M) == (256, 128))
+    g, s, u, _ = inst[k]
+    s0 = s.astype(np.float32).astype(np.float64)
+    u0 = u.astype(np.float32).astype(np.float64)
+    ref = ora.Sim(g, s0, u0, G.HLLC).advance(steps)
+    e_r, e_p = _l1(states[k][0][..., 0], ref.s[..., 0]), _l1(states[k][0][..., 1], ref.s[..., 1])
+    print(f"  instance {k} vs oracle: L1 rho {e_r:.2e}  L1 p {e_p:.2e}")
+    assert e_r <= 1e-5 and e_p <= 1e-5
+
+
+def _parity_grids():
+    base = G.Grid.from_extent(37, 21, -1.0, 2.0, 1.5, gamma=5.0 / 3.0, krit=0.3)
+    yield "planar-ref-bcs", base.replace(geometry=G.PLANAR)
+    yield "axi-outflow", base.replace(geometry=G.AXISYMMETRIC,
+                                      bc=(G.BC_OUTFLOW, G.BC_OUTFLOW, G.BC_REFLECT, G.BC_OUTFLOW))
+    yield "clamped+dirichlet", base.replace(
+        geometry=G.AXISYMMETRIC, bc=(G.BC_OUTFLOW_CLAMPED, G.BC_DIRICHLET, G.BC_REFLECT,
+                                     G.BC_OUTFLOW_CLAMPED),
+        bc_state=[[0.0] * 4, [0.5, 0.3, -0.7, 0.1], [0.0] * 4, [0.0] * 4],
+        bc_vclamp=(-0.2, 0.0, 0.0, 0.15))
+    yield "window+frozen", base.replace(geometry=G.AXISYMMETRIC, ho_lo=-0.3, ho_hi=1.1,
+                                        frozen_x0=0.4, frozen_ax=0.5, frozen_ay=0.4)
+
+
+@pytest.mark.parametrize("method", [G.HLL, G.HLLC, G.EXACT, G.HLLC_A])
+@pytest.mark.parametrize("name,grid", list(_parity_grids()), ids=[n for n, _ in _parity_grids()])
+def test_fp32_step_vs_oracle(dev, ora, name, grid, method):
+    """One FP32 step against the oracle's step from the same (FP32-rounded)
+    state: every cell within 3e-6 of the oracle (rho, p relative; velocities
+    against |u| + c), and the same dt to FP32 accuracy."""
+    import torch
+    from asteriy_2d_amd import solver
+    print(f"FP32 step {name} method={method}", flush=True)
+    s, u = problems.random_state(grid, seed=9)
+    s = s.astype(np.float32).astype(np.float64)
+    u = u.astype(np.float32).astype(np.float64)
+    T_do = 0.01
+    S, U = _t(s, dev, torch.float32), _t(u, dev, torch.float32)
+    S2, U2 = torch.empty_like(S), torch.empty_like(U)
+    T = torch.tensor([1e7], dtype=torch.float64, device=dev)
+    Td = torch.tensor([T_do], dtype=torch.float64, device=dev)
+    TT = torch.zeros(1, dtype=torch.float64, device=dev)
+    solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method)
+    torch.cuda.synchronize()
+    rs2, ru2, rT = ora.step(grid, s, u, 1e7, T_do, method)
+    bad, worst = step_mismatch(S2.double().cpu().numpy(), U2.double().cpu().numpy(), rs2, ru2,
+                               grid.gamma, rel=3e-6)
+    print(f"  worst relative difference {worst:.2e}")
+    assert bad == 0, (bad, worst)
+    assert abs(float(T.item()) - rT) <= 3e-6 * rT
