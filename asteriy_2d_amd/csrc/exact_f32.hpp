@@ -277,6 +277,34 @@ AST_DEV R shock_raref(R g, R g1, R gp, R gm, R enI, R pI, R rI, R enII, R pII, R
   return p;
 }
 
+// The wave pattern exact_f32 will find for a face, as a schedule key for
+// sorting faces by pattern: 0 vacuum, 1 two rarefactions, 2 shock +
+// rarefaction, 3 two shocks, 4 two shocks that are strong (acoustic estimate
+// above kStrongShock * pII: the learned start's faces), 5 no pattern (NaN).
+// The same tests as exact_f32's, so the key is the pattern the solver takes.
+constexpr int kPatternKeys = 6;
+AST_DEV int pattern_key(const PrimF& L, const PrimF& R, const ExactCtx& e) {
+  const bool sw = L.p > R.p;
+  const float enI = sw ? -R.u : L.u, enII = sw ? -L.u : R.u;
+  const float pI = sw ? R.p : L.p, pII = sw ? L.p : R.p;
+  const float rI = sw ? R.r : L.r, rII = sw ? L.r : R.r;
+  const float cI = fsqrt(e.g * pI * frcp(rI)), cII = fsqrt(e.g * pII * frcp(rII));
+  const float q = fpow(pI * frcp(pII), e.gm);
+  const float a = fsqrt(rI * (e.g2 * pII + e.g1 * pI) * 0.5f);
+  const float Uud = (pII - pI) * frcp(a);
+  const float Urz = -e.c2g1 * cII * (1.f - q);
+  const float Uvk = -e.c2g1 * (cII + cI);
+  const float Udf = enI - enII;
+  if (Udf < Uvk) return 0;
+  if (Udf >= Uvk && Udf <= Urz) return 1;
+  if (Udf > Urz && Udf <= Uud) return 2;
+  if (Udf > Uud) {
+    const float zI = rI * cI, zII = rII * cII;
+    return (pI * zII + pII * zI + Udf * zI * zII) * frcp(zI + zII) > kStrongShock * pII ? 4 : 3;
+  }
+  return 5;
+}
+
 // FP64 sound speed of an FP32 state (for the FP64-formed differences)
 AST_DEV double sound_speed_d(double g, float p, float r) { return sqrt(g * (double)p / (double)r); }
 

@@ -415,6 +415,26 @@ AST_HD FluxD exact(const PrimD& L, const PrimD& R, const GasD& a, double& UU, Ex
   return f;
 }
 
+// The wave pattern `exact` will take for a face, as a schedule key for
+// sorting faces by pattern: 0 vacuum, 1 two rarefactions, 2 shock +
+// rarefaction, 3 two shocks, 4 no pattern (NaN) -- the same tests.
+constexpr int kPatternKeys = 5;
+AST_HD int pattern_key(const PrimD& L, const PrimD& R, const GasD& a) {
+  const bool sw = L.p > R.p;
+  const double rI = sw ? R.r : L.r, pI = sw ? R.p : L.p, uI = sw ? -R.u : L.u;
+  const double rII = sw ? L.r : R.r, pII = sw ? L.p : R.p, uII = sw ? -L.u : R.u;
+  const double cI = sqrt(a.g * pI / rI), cII = sqrt(a.g * pII / rII);
+  const double du = uI - uII;
+  const double du_vac = -a.c2g1 * (cI + cII);
+  const double du_rr = a.c2g1 * cII * (pow(pI / pII, a.gm) - 1.0);
+  const double du_sr = (pII - pI) / sqrt(0.5 * rI * (a.g2 * pII + a.g1 * pI));
+  if (du < du_vac) return 0;
+  if (du >= du_vac && du <= du_rr) return 1;
+  if (du > du_rr && du <= du_sr) return 2;
+  if (du > du_sr) return 3;
+  return 4;
+}
+
 // The method's flux (AST_HLL, AST_HLLC, AST_HLLC_A, AST_EXACT).
 template <int METHOD>
 AST_HD FluxD face_flux(const PrimD& L, const PrimD& R, const GasD& a, double& UU,

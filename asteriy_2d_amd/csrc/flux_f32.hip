@@ -67,28 +67,182 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
   if constexpr (STATS) block_stats_f32_to<FNT>(st, redi, stats);
 }
 
+// ---- the exact solver on pattern-sorted faces ------------------------------
+//
+// Faces of one wave that take different wave patterns (and different
+// iteration counts) run their branches one after another with most lanes
+// masked.  Here each workgroup takes a tile of STILE faces, classifies them
+// (pattern_key: vacuum, two rarefactions, shock + rarefaction, two shocks,
+// strong two shocks, no pattern), orders the tile's faces by that key with a
+// counting sort in LDS, and then solves them in that order, so that the 64
+// lanes of a wave mostly hold faces of one pattern.  Results depend only on
+// each face's own data: the order changes no bit.
+//
+// With the learned newton start, a first kernel (kGuessPass) evaluates the
+// network for the strong two-shock faces only -- they are contiguous after
+// the sort, so its waves are full of faces that use the guess -- and leaves
+// each guess in the face's dt slot of `out`; the solve kernel
+// (kSolveGuessed) reads it there before it writes the dt.  The solver kernel
+// thus carries no network registers, and the network's own kernel runs only
+// on the faces it serves.
+constexpr int SORT_NT = 256;
+constexpr int SORT_PER = 4;                  // faces per thread per tile
+constexpr int STILE = SORT_NT * SORT_PER;    // faces per tile
+constexpr int kSolve = 0, kGuessPass = 1, kSolveGuessed = 2;
+
+AST_DEV void load_face32(const float* __restrict__ in, size_t n, size_t i, float n1, float n2,
+                         PrimF& L, PrimF& R) {
+  const float vxl = in[2 * n + i], vyl = in[3 * n + i];
+  const float vxr = in[6 * n + i], vyr = in[7 * n + i];
+  L = PrimF{in[i], in[n + i], vxl * n1 + vyl * n2, -vxl * n2 + vyl * n1};
+  R = PrimF{in[4 * n + i], in[5 * n + i], vxr * n1 + vyr * n2, -vxr * n2 + vyr * n1};
+}
+
+template <bool STATS, int MODE>
+__global__ __launch_bounds__(SORT_NT) void exact_sorted_kernel(size_t n,
+                                                               const float* __restrict__ in,
+                                                               float n1, float n2, float krad,
+                                                               ExactCtx ex, float* out,
+                                                               ast_stats_t* stats,
+                                                               int* __restrict__ info) {
+  constexpr bool STRONG = MODE != kSolve;  // strong faces form a group of their own
+  __shared__ uint32_t redi[(SORT_NT / 64) * 10];
+  __shared__ unsigned short perm[STILE];
+  __shared__ int cnt[kPatternKeys + 1], cur[kPatternKeys + 1];
+  typename std::conditional<STATS, StatsF32, NoStats>::type st;
+  st.clear();
+  const size_t ntiles = (n + STILE - 1) / STILE;
+  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {  // workgroup-uniform
+    const size_t base = tile * STILE;
+    const int nv = (int)min((size_t)STILE, n - base);
+    if (threadIdx.x <= kPatternKeys) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    // classify (coalesced loads) and count
+    int key[SORT_PER];
+#pragma unroll
+    for (int j = 0; j < SORT_PER; j++) {
+      const int li = j * SORT_NT + threadIdx.x;
+      key[j] = kPatternKeys;  // past the end of the batch
+      if (li < nv) {
+        PrimF L, R;
+        load_face32(in, n, base + li, n1, n2, L, R);
+        key[j] = pattern_key(L, R, ex);
+        if (!STRONG && key[j] == 4) key[j] = 3;  // no network: strong faces are plain ones
+      }
+      atomicAdd(&cnt[key[j]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int k = 0; k <= kPatternKeys; k++) {
+        cur[k] = s;
+        s += cnt[k];
+      }
+      cnt[0] = cur[4];             // first strong face
+      cnt[1] = cur[5];             // one past the last strong face
+      cnt[2] = cur[kPatternKeys];  // the tile's faces of the batch
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SORT_PER; j++) {
+      const int slot = atomicAdd(&cur[key[j]], 1);
+      perm[slot] = (unsigned short)(j * SORT_NT + threadIdx.x);
+    }
+    __syncthreads();
+    const int s0 = cnt[0], s1 = cnt[1], nfaces = cnt[2];
+    if constexpr (MODE == kGuessPass) {
+      // the network on the strong faces, whole waves of them (EXEC full)
+      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      for (int w0 = s0 + 64 * wave; w0 < s1; w0 += SORT_NT) {  // wave-uniform
+        const int li = perm[min(w0 + lane, s1 - 1)];
+        PrimF L, R;
+        load_face32(in, n, base + li, n1, n2, L, R);
+        const bool sw = L.p > R.p;
+        float fe[mlp::IN];
+        const float lp = mlp::features<float>(sw ? R.r : L.r, sw ? R.p : L.p, sw ? -R.u : L.u,
+                                              sw ? L.r : R.r, sw ? L.p : R.p, sw ? -L.u : R.u,
+                                              ex.g, fe);
+        const float y = mlp::forward_wave_bf16(ex.mlp, fe);
+        if (w0 + lane < s1) out[4 * n + base + li] = __expf(y + lp);
+      }
+    } else {
+      // solve in sorted order
+#pragma unroll 1
+      for (int j = 0; j < SORT_PER; j++) {  // uniform
+        const int pos = j * SORT_NT + threadIdx.x;
+        FaceInfo fi{kNoFace, 0, false, false, false, false};
+        if (pos < nfaces) {
+          const int li = perm[pos];
+          const size_t i = base + li;
+          PrimF L, R;
+          load_face32(in, n, i, n1, n2, L, R);
+          const float g = (MODE == kSolveGuessed && pos >= s0 && pos < s1) ? out[4 * n + i] : -1.f;
+          float UU;
+          const FluxF f = exact_f32<false>(L, R, ex, UU, fi, g);
+          out[i] = f.m;
+          out[n + i] = f.e;
+          out[2 * n + i] = f.un * n1 - f.ut * n2;
+          out[3 * n + i] = f.un * n2 + f.ut * n1;
+          out[4 * n + i] = krad * frcp(UU);
+          if (info) {
+            info[2 * i] = fi.it;
+            info[2 * i + 1] = (fi.pat == kNoPattern ? -1 : fi.pat + 1) | (fi.capped ? 8 : 0) |
+                              (fi.island ? 16 : 0);
+          }
+        }
+        st.add(fi);
+      }
+    }
+    __syncthreads();  // perm and the counters are reused by the next tile
+  }
+  if constexpr (STATS) block_stats_f32_to<SORT_NT>(st, redi, stats);
+}
+
+template <int METHOD>
+static int launch_unsorted(size_t n, const float* in, float n1, float n2, float krad,
+                           const ExactCtx& ex, float* out, ast_stats_t* stats, int* info,
+                           hipStream_t stream);
+
 template <int METHOD>
 static int launch(size_t n, const float* in, float n1, float n2, float krad, const ExactCtx& ex,
                   float* out, ast_stats_t* stats, int* info, hipStream_t stream) {
+  if constexpr (METHOD == AST_EXACT) {
+    size_t blocks = (n + STILE - 1) / STILE;
+    if (blocks > 256 * 16) blocks = 256 * 16;
+    const dim3 gr((unsigned)blocks), bl(SORT_NT);
+    const bool st = stats != nullptr;
+    if (ex.mlp != nullptr) {
+      hipLaunchKernelGGL((exact_sorted_kernel<false, kGuessPass>), gr, bl, 0, stream, n, in, n1,
+                         n2, krad, ex, out, nullptr, nullptr);
+      AST_LAUNCH_CHECK("ast_flux_f32 guess launch");
+      if (st)
+        hipLaunchKernelGGL((exact_sorted_kernel<true, kSolveGuessed>), gr, bl, 0, stream, n, in,
+                           n1, n2, krad, ex, out, stats, info);
+      else
+        hipLaunchKernelGGL((exact_sorted_kernel<false, kSolveGuessed>), gr, bl, 0, stream, n, in,
+                           n1, n2, krad, ex, out, stats, info);
+    } else if (st) {
+      hipLaunchKernelGGL((exact_sorted_kernel<true, kSolve>), gr, bl, 0, stream, n, in, n1, n2,
+                         krad, ex, out, stats, info);
+    } else {
+      hipLaunchKernelGGL((exact_sorted_kernel<false, kSolve>), gr, bl, 0, stream, n, in, n1, n2,
+                         krad, ex, out, stats, info);
+    }
+    AST_LAUNCH_CHECK("ast_flux_f32 launch");
+    return AST_OK;
+  } else {
+    return launch_unsorted<METHOD>(n, in, n1, n2, krad, ex, out, stats, info, stream);
+  }
+}
+
+template <int METHOD>
+static int launch_unsorted(size_t n, const float* in, float n1, float n2, float krad,
+                           const ExactCtx& ex, float* out, ast_stats_t* stats, int* info,
+                           hipStream_t stream) {
   size_t blocks = (n + FNT - 1) / FNT;
   if (blocks > 256 * 64) blocks = 256 * 64;
-  const bool st = METHOD == AST_EXACT && stats != nullptr;
-  if (METHOD == AST_EXACT && ex.mlp != nullptr) {
-    if constexpr (METHOD == AST_EXACT) {
-      if (st)
-        hipLaunchKernelGGL((flux_kernel<METHOD, true, true>), dim3((unsigned)blocks), dim3(FNT), 0,
-                           stream, n, in, n1, n2, krad, ex, out, stats, info);
-      else
-        hipLaunchKernelGGL((flux_kernel<METHOD, false, true>), dim3((unsigned)blocks), dim3(FNT), 0,
-                           stream, n, in, n1, n2, krad, ex, out, stats, info);
-    }
-  } else if (st) {
-    hipLaunchKernelGGL((flux_kernel<METHOD, true, false>), dim3((unsigned)blocks), dim3(FNT), 0,
-                       stream, n, in, n1, n2, krad, ex, out, stats, info);
-  } else {
-    hipLaunchKernelGGL((flux_kernel<METHOD, false, false>), dim3((unsigned)blocks), dim3(FNT), 0,
-                       stream, n, in, n1, n2, krad, ex, out, stats, info);
-  }
+  hipLaunchKernelGGL((flux_kernel<METHOD, false, false>), dim3((unsigned)blocks), dim3(FNT), 0,
+                     stream, n, in, n1, n2, krad, ex, out, stats, info);
   AST_LAUNCH_CHECK("ast_flux_f32 launch");
   return AST_OK;
 }

@@ -92,6 +92,78 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const double* __res
   if (METHOD == AST_EXACT && stats != nullptr) block_stats_to(st, redi, stats);
 }
 
+// The exact solver on pattern-sorted faces: each workgroup classifies a tile
+// of STILE faces (pattern_key), orders them by pattern with a counting sort
+// in LDS and solves them in that order, so that a wave's lanes mostly take
+// the same wave pattern and iteration loop (the faces of one wave otherwise
+// run the branches of every pattern one after another).  Each face's result
+// depends only on its own data.
+constexpr int SORT_PER = 4;
+constexpr int STILE = FNT * SORT_PER;
+
+__global__ __launch_bounds__(FNT) void exact_sorted_kernel(size_t n, const double* __restrict__ in,
+                                                           double n1, double n2, double krad,
+                                                           GasD gas, double* __restrict__ out,
+                                                           ast_stats_t* stats) {
+  __shared__ uint32_t redi[(FNT / 64) * 10];
+  __shared__ unsigned short perm[STILE];
+  __shared__ int cnt[kPatternKeys + 1], cur[kPatternKeys + 1];
+  LaneStats st;
+  st.clear();
+  const size_t ntiles = (n + STILE - 1) / STILE;
+  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {  // workgroup-uniform
+    const size_t base = tile * STILE;
+    const int nv = (int)min((size_t)STILE, n - base);
+    if (threadIdx.x <= kPatternKeys) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int key[SORT_PER];
+#pragma unroll
+    for (int j = 0; j < SORT_PER; j++) {
+      const int li = j * FNT + threadIdx.x;
+      key[j] = kPatternKeys;  // past the end of the batch
+      if (li < nv) {
+        PrimD L, R;
+        load_face(in, n, base + li, n1, n2, L, R);
+        key[j] = pattern_key(L, R, gas);
+      }
+      atomicAdd(&cnt[key[j]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int k = 0; k <= kPatternKeys; k++) {
+        cur[k] = s;
+        s += cnt[k];
+      }
+      cnt[0] = cur[kPatternKeys];  // the tile's faces of the batch
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SORT_PER; j++) {
+      const int slot = atomicAdd(&cur[key[j]], 1);
+      perm[slot] = (unsigned short)(j * FNT + threadIdx.x);
+    }
+    __syncthreads();
+    const int nfaces = cnt[0];
+#pragma unroll 1
+    for (int j = 0; j < SORT_PER; j++) {
+      const int pos = j * FNT + threadIdx.x;
+      if (pos < nfaces) {
+        const size_t i = base + perm[pos];
+        PrimD L, R;
+        load_face(in, n, i, n1, n2, L, R);
+        double UU;
+        ExactInfo ei;
+        const FluxD f = exact(L, R, gas, UU, ei);
+        st.add(ei);
+        store_face<AST_EXACT>(out, n, i, n1, n2, f, krad, UU);
+      }
+    }
+    __syncthreads();  // perm and the counters are reused by the next tile
+  }
+  if (stats != nullptr) block_stats_to(st, redi, stats);
+}
+
 // The exact solver with per-face newton starting points and diagnostics.
 __global__ __launch_bounds__(FNT) void guess_kernel(size_t n, const double* __restrict__ in,
                                                     double n1, double n2, double krad, GasD gas,
@@ -130,10 +202,19 @@ static unsigned flux_blocks(size_t n) {
 template <int METHOD>
 static int launch_flux(size_t n, const double* in, double n1, double n2, double krad,
                        const GasD& gas, double* out, ast_stats_t* stats, hipStream_t stream) {
-  hipLaunchKernelGGL(flux_kernel<METHOD>, dim3(flux_blocks(n)), dim3(FNT), 0, stream, n, in, n1,
-                     n2, krad, gas, out, stats);
-  AST_LAUNCH_CHECK("ast_flux_f64 launch");
-  return AST_OK;
+  if constexpr (METHOD == AST_EXACT) {
+    size_t blocks = (n + STILE - 1) / STILE;
+    if (blocks > 256 * 16) blocks = 256 * 16;
+    hipLaunchKernelGGL(exact_sorted_kernel, dim3((unsigned)blocks), dim3(FNT), 0, stream, n, in,
+                       n1, n2, krad, gas, out, stats);
+    AST_LAUNCH_CHECK("ast_flux_f64 launch");
+    return AST_OK;
+  } else {
+    hipLaunchKernelGGL(flux_kernel<METHOD>, dim3(flux_blocks(n)), dim3(FNT), 0, stream, n, in,
+                       n1, n2, krad, gas, out, stats);
+    AST_LAUNCH_CHECK("ast_flux_f64 launch");
+    return AST_OK;
+  }
 }
 
 }  // namespace f64

@@ -216,6 +216,28 @@ def _run_sim(grid, method, precision, s, u, steps, mlp=None, autotune=False, sta
     return ms, st, t, counters, rows
 
 
+def _fp64_fp32_state(grid, s, u, steps):
+    """`steps` FP64 HLLC steps (the reference protocol: step, funk_time) with
+    the state rounded to FP32 after every step; returns the final state."""
+    import torch
+
+    from asteriy_2d_amd import HLLC, solver
+    dev = torch.device("cuda:0")
+    S = torch.from_numpy(s).to(dev).float().double()
+    U = torch.from_numpy(u).to(dev).float().double()
+    S2, U2 = torch.empty_like(S), torch.empty_like(U)
+    T = torch.tensor([1e7], dtype=torch.float64, device=dev)
+    Td = torch.tensor([1e-9], dtype=torch.float64, device=dev)
+    TT = torch.zeros(1, dtype=torch.float64, device=dev)
+    for _ in range(steps):
+        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, HLLC)
+        solver.funk_time(T, Td, TT)
+        S.copy_(S2.float())
+        U.copy_(U2.float())
+    torch.cuda.synchronize()
+    return S.cpu().numpy(), U.cpu().numpy()
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -423,6 +445,9 @@ def main(argv=None):
         _, a32, _, _, _ = _run_sim(g, HLLC, "f32", p.s, p.u, 300)
         _, a64, _, _, _ = _run_sim(g, HLLC, "f64", p.s, p.u, 300)
         extra["error_vs_fp64_300_steps"] = _errors(a32, a64)
+        # the FP64 step with its state rounded to FP32 after every step: the
+        # error FP32 storage alone causes, whatever the arithmetic
+        extra["fp64_fp32_state_vs_fp64"] = _errors(_fp64_fp32_state(g, p.s, p.u, K), r64)
     del r64
 
     if not a.quick:
