@@ -50,10 +50,17 @@ extern "C" int ast_grid_check(const ast_grid_t* g) {
   if (g->geometry == AST_AXISYMMETRIC && !(g->y_min > 0))
     return ast_fail(AST_ERR_ARG, "grid: axisymmetric update needs cell radii y > 0 (y_min > 0)");
   for (int i = 0; i < 4; i++) {
-    if (g->bc[i] < AST_BC_OUTFLOW || g->bc[i] > AST_BC_OUTFLOW_CLAMPED)
+    if (g->bc[i] < AST_BC_OUTFLOW || g->bc[i] > AST_BC_DIRICHLET_PULSED)
       return ast_fail(AST_ERR_ARG, "grid: bc[%d] = %d unknown", i, g->bc[i]);
-    if (g->bc[i] == AST_BC_DIRICHLET && !(g->bc_state[i][0] > 0 && g->bc_state[i][1] > 0))
+    if ((g->bc[i] == AST_BC_DIRICHLET || g->bc[i] == AST_BC_DIRICHLET_PULSED) &&
+        !(g->bc_state[i][0] > 0 && g->bc_state[i][1] > 0))
       return ast_fail(AST_ERR_ARG, "grid: Dirichlet state of side %d needs rho, p > 0", i);
+    if (g->bc[i] == AST_BC_DIRICHLET_PULSED &&
+        !(fabs(g->bc_pulse[0]) < 1.0 && isfinite(g->bc_pulse[1]) && !isnan(g->bc_pulse[2]) &&
+          !isnan(g->bc_pulse[3])))
+      return ast_fail(AST_ERR_ARG,
+                      "grid: pulsed side %d needs |a| < 1 (a positive density), a finite w and "
+                      "limits that are not NaN", i);
   }
   if (g->frozen_ax > 0 && !(g->frozen_ay > 0))
     return ast_fail(AST_ERR_ARG, "grid: frozen region needs frozen_ay > 0");

@@ -36,14 +36,22 @@ AST_DEV int xcd_tile(int bid, int ntiles) {
   return xcd * q + (xcd < r ? xcd : r) + local;
 }
 
+// Density factor of AST_BC_DIRICHLET_PULSED sides at simulation time TT
+// (add2's inflow perturbation, kernel.cu:2022-2025): 1 + a sin(w TT) while
+// TT < t_lim, else 1.
+AST_DEV double pulse_factor(const ast_grid_t& g, double TT) {
+  return TT < g.bc_pulse[3] ? 1.0 + g.bc_pulse[0] * sin(TT * g.bc_pulse[1]) : 1.0;
+}
+
 // Cell (n, m) of a state plane with the ghost rules of Kernel_TVD
 // (kernel.cu:2549-2698) and add_MK (kernel.cu:2328-2377).  Indices may lie
 // up to 2 cells outside the grid on at most one axis; anything else
 // (corners, beyond the halo of a partial tile) is never read by a cell and
-// returns zeros.
+// returns zeros.  pulse: the density factor of pulsed Dirichlet ghosts whose
+// centre lies below y_lim (pulse_factor).
 template <typename P>
 AST_DEV void fetch_cell(const ast_grid_t& g, const P* __restrict__ s, const P* __restrict__ u,
-                        int n, int m, P& so, P& uo) {
+                        int n, int m, P& so, P& uo, double pulse = 1.0) {
   const int N = g.N, M = g.M;
   const bool xin = (n >= 0) & (n < N), yin = (m >= 0) & (m < M);
   if (xin & yin) {
@@ -59,11 +67,13 @@ AST_DEV void fetch_cell(const ast_grid_t& g, const P* __restrict__ s, const P* _
   else if (m < 0) { side = AST_YLO; in_m = -1 - m; }
   else { side = AST_YHI; in_m = 2 * M - 1 - m; }
   const int bc = g.bc[side];
-  if (bc == AST_BC_DIRICHLET) {
+  if (bc == AST_BC_DIRICHLET || bc == AST_BC_DIRICHLET_PULSED) {
     so.x = g.bc_state[side][0];
     so.y = g.bc_state[side][1];
     uo.x = g.bc_state[side][2];
     uo.y = g.bc_state[side][3];
+    if (bc == AST_BC_DIRICHLET_PULSED && g.y_min + m * g.dy < g.bc_pulse[2])
+      so.x = (decltype(so.x))(g.bc_state[side][0] * pulse);
     return;
   }
   if (bc == AST_BC_REFLECT) {

@@ -39,11 +39,8 @@ struct ExactCtx {
   int n;  // 1/gm = 2 gamma/(gamma - 1) when that is an integer in [1, 16] (5 for
           // gamma = 5/3, 7 for 7/5), else 0
   const unsigned char* mlp;  // learned-guess block (ast_mlp_t) in device memory, or null
-  // Gated launch (the runtime's learned mode): the kernel runs only while
-  // *gate == gate_mode, and ORs 1 into *strong when it meets a strong
-  // two-shock face (null: ungated, nothing flagged).
-  const uint32_t* gate;
-  uint32_t gate_mode;
+  // When set, the kernel ORs 1 into *strong when it meets a strong two-shock
+  // face (the runtime's learned mode, time_slots.h).
   uint32_t* strong;
 };
 
@@ -66,8 +63,6 @@ static inline ExactCtx make_exact_ctx(double gamma, const void* mlp) {
   // (an FP32 gamma of 5/3 or 7/5 still counts: 1e-6 is its resolution)
   e.n = (fabs(ig - ni) <= 1e-6 * ig && ni >= 1.0 && ni <= 16.0) ? (int)ni : 0;
   e.mlp = (const unsigned char*)mlp;
-  e.gate = nullptr;
-  e.gate_mode = 0;
   e.strong = nullptr;
   return e;
 }

@@ -120,8 +120,14 @@ AST_HD FluxD hll(const PrimD& L, const PrimD& R, const GasD& a, double& UU) {
 // S_K - u_K is then formed without the cancellation of S_K next to u_K on a
 // cold side: S_L - u_L = min(-c_L, (u_R - u_L) - c_R), S_R - u_R =
 // max(c_L - (u_R - u_L), c_R).
-template <bool PER_SIDE>
-AST_HD FluxD hllc(const PrimD& L, const PrimD& R, const GasD& a, double& UU) {
+//
+// SCALAR adds a passive scalar Q (a conserved density, carried at Q/rho):
+// its flux *PQ is Q_K u_K outside the waves and Q*_K S_M in the star region,
+// Q*_K = Q_K (S_K - u_K)/(S_K - S_M) (Q_LL, Q_RR of Gas_Dinamic.cu:1700-1701;
+// the star values of HLLCQ_Korolkov_2D, Gas_Dinamic.cu:305-331).
+template <bool PER_SIDE, bool SCALAR = false>
+AST_HD FluxD hllc(const PrimD& L, const PrimD& R, const GasD& a, double& UU, double QL = 0.0,
+                  double QR = 0.0, double* PQ = nullptr) {
   double SL, SR, wL, wR;  // wave speeds and S_K - u_K
   if (PER_SIDE) {
     const double cL = sqrt(a.g * L.p / L.r), cR = sqrt(a.g * R.p / R.r);
@@ -146,9 +152,13 @@ AST_HD FluxD hllc(const PrimD& L, const PrimD& R, const GasD& a, double& UU) {
   const bool left = supL || (!supR && SM >= 0.0);
   const PrimD& K = left ? L : R;
   const double EK = total_energy(K, a.ig1);
-  if (supL || supR) return side_flux(K, EK);
+  if (supL || supR) {
+    if constexpr (SCALAR) *PQ = (left ? QL : QR) * K.u;
+    return side_flux(K, EK);
+  }
   const double SK = left ? SL : SR, mK = left ? mL : mR, wK = left ? wL : wR;
   const double dK = 1.0 / (SK - SM);
+  if constexpr (SCALAR) *PQ = (left ? QL : QR) * wK * dK * SM;
   const double smK = SM - K.u;
   const double q = mK * smK;  // p* - p_K
   const double ps = K.p + q;

@@ -191,6 +191,23 @@ __global__ __launch_bounds__(FNT) void guess_kernel(size_t n, const double* __re
   if (stats != nullptr) block_stats_to(st, redi, stats);
 }
 
+// HLLC with a passive scalar: 10 input planes (the 8 of load_face, Q_L,
+// Q_R) and 6 output planes (the 5 of store_face, PQ); 128 B per face.
+template <bool PER_SIDE>
+__global__ __launch_bounds__(FNT) void flux_q_kernel(size_t n, const double* __restrict__ in,
+                                                     double n1, double n2, double krad, GasD gas,
+                                                     double* __restrict__ out) {
+  const size_t stride = (size_t)gridDim.x * FNT;
+  for (size_t i = (size_t)blockIdx.x * FNT + threadIdx.x; i < n; i += stride) {
+    PrimD L, R;
+    load_face(in, n, i, n1, n2, L, R);
+    double UU, PQ;
+    const FluxD f = hllc<PER_SIDE, true>(L, R, gas, UU, in[8 * n + i], in[9 * n + i], &PQ);
+    store_face<AST_HLLC>(out, n, i, n1, n2, f, krad, UU);
+    out[5 * n + i] = PQ;
+  }
+}
+
 static unsigned flux_blocks(size_t n) {
   // enough workgroups to fill 256 CUs several times over; the loop covers
   // the rest
@@ -250,5 +267,26 @@ extern "C" int ast_flux_f64_guess(size_t n, const double* in, double n1, double 
   hipLaunchKernelGGL(guess_kernel, dim3(flux_blocks(n)), dim3(FNT), 0, (hipStream_t)stream, n, in,
                      n1, n2, krit * rad, gas, guess, out, stats, info);
   AST_LAUNCH_CHECK("ast_flux_f64_guess launch");
+  return AST_OK;
+}
+
+extern "C" int ast_flux_q_f64(int method, size_t n, const double* in, double n1, double n2,
+                              double rad, double gamma, double krit, double* out, void* stream) {
+  using namespace ast::f64;
+  if (n == 0) return AST_OK;
+  if (!in || !out) return ast_fail(AST_ERR_ARG, "ast_flux_q_f64: null in/out");
+  if (!(gamma > 1.0)) return ast_fail(AST_ERR_ARG, "ast_flux_q_f64: gamma must be > 1");
+  const GasD gas = ast::d64::make_gas(gamma);
+  hipStream_t st = (hipStream_t)stream;
+  if (method == AST_HLLC)
+    hipLaunchKernelGGL(flux_q_kernel<false>, dim3(flux_blocks(n)), dim3(FNT), 0, st, n, in, n1,
+                       n2, krit * rad, gas, out);
+  else if (method == AST_HLLCQ || method == AST_HLLCQ_A)
+    hipLaunchKernelGGL(flux_q_kernel<true>, dim3(flux_blocks(n)), dim3(FNT), 0, st, n, in, n1,
+                       n2, krit * rad, gas, out);
+  else
+    return ast_fail(AST_ERR_ARG, "ast_flux_q_f64: method %d (AST_HLLC, AST_HLLCQ, AST_HLLCQ_A)",
+                    method);
+  AST_LAUNCH_CHECK("ast_flux_q_f64 launch");
   return AST_OK;
 }

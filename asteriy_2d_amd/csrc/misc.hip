@@ -18,14 +18,14 @@ __global__ void funk_time_kernel(double* T, double* T_do, double* TT, int* i) {
 
 // The time advance that closes a run of steps on the runtime's time block
 // (time_slots.h): funk_time with T = the minimum of the slots of set `last`
-// (the last step's), the learned mode of the next step from that set's
-// strong flag, and every set cleared for the next run.  One wave; lane k
+// (the last step's), and every set cleared for the next run.  With `flags`
+// (pinned host memory, 2 words), the run's strong-face flag goes to
+// flags[run & 1] and is cleared, and the run is counted.  One wave; lane k
 // folds and clears slot k.
-__global__ void funk_time_fold_kernel(double* T, int last, int nslots) {
+__global__ void funk_time_fold_kernel(double* T, int last, int nslots, uint32_t* flags) {
   const int k = threadIdx.x;
   double* L = T + kSlot0 + kSetDoubles * last;
   double t = k < nslots ? L[kSlotStride * k] : INFINITY;
-  const uint32_t strong = reinterpret_cast<const uint32_t*>(L + 1)[0];
   t = wave_min(t);
   if (k == 0) {
     t = fmin(t, *T);  // a step launched without slots lowers T itself
@@ -34,10 +34,14 @@ __global__ void funk_time_fold_kernel(double* T, int last, int nslots) {
     T[0] = kSlotEmpty;
     int* i = reinterpret_cast<int*>(T + 3);
     *i = *i + 1;
-    // learned mode of the next step: on while strong two-shock faces occur
-    uint32_t* gate = reinterpret_cast<uint32_t*>(T + kGateWord);
-    gate[0] = strong ? 1u : 0u;
-    gate[1] = 0u;
+    uint32_t* w = reinterpret_cast<uint32_t*>(T + kGateWord);  // {strong, run}
+    if (flags != nullptr) {
+      const uint32_t run = w[1];
+      __atomic_store_n(&flags[run & 1u], w[0], __ATOMIC_RELAXED);
+      __threadfence_system();
+      w[0] = 0u;
+      w[1] = run + 1u;
+    }
   }
   for (int j = 0; j < kSets; j++) {
     double* S = T + kSlot0 + kSetDoubles * j;
@@ -133,11 +137,11 @@ __global__ void sums_pass2(const double* work, double* out) {
 }  // namespace ast
 
 // The time advance closing a run whose last step lowered slot set `last`.
-int ast_funk_time_fold(double* T, int last, int nslots, void* stream) {
+int ast_funk_time_fold(double* T, int last, int nslots, uint32_t* flags, void* stream) {
   if (!T || last < 0 || nslots < 1 || nslots > ast::kMaxSlots)
     return ast_fail(AST_ERR_ARG, "ast_funk_time_fold: bad argument");
   hipLaunchKernelGGL(ast::funk_time_fold_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, T,
-                     last % ast::kSets, nslots);
+                     last % ast::kSets, nslots, flags);
   AST_LAUNCH_CHECK("ast_funk_time_fold launch");
   return AST_OK;
 }

@@ -28,10 +28,10 @@ int ast_hip_fail(hipError_t e, const char* where);
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
                       ast_stats_t* stats, int krows, const ast::StepTime* tm, void* stream);
-int ast_funk_time_fold(double* T, int last, int nslots, void* stream);
+int ast_funk_time_fold(double* T, int last, int nslots, uint32_t* flags, void* stream);
 int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, double* s2,
-                      double* u2, double* T, const double* T_do, int method, ast_stats_t* stats,
-                      int rows, void* stream);
+                      double* u2, double* T, const double* T_do, const double* TT, int method,
+                      ast_stats_t* stats, int rows, const ast_sources_t* src, void* stream);
 extern "C" int ast_get_tile_rows(void);
 
 #define CK(call)                                              \
@@ -55,9 +55,24 @@ struct ast_sim {
   bool collect;  // count the exact solver's work into stats (counting kernels)
   const ast_mlp_t* mlp;
   hipStream_t stream;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  bool graph_ok;
+  hipGraph_t graph[2];      // a captured run of kChunk steps, per learned mode
+  hipGraphExec_t exec[2];
+  bool graph_ok[2];
+  // learned mode (FP32 exact solver with a learned guess): each run of steps
+  // uses one solver kernel, the plain one (which flags strong two-shock
+  // faces) or the one with the learned start; the kernel that closes run k
+  // writes whether it met strong faces to hflags[k & 1] (pinned host
+  // memory), and run k + 2 takes that as its mode, after waiting for run k's
+  // completion event -- the host never waits for the run in flight
+  uint32_t* hflags;         // pinned host, 2 words (fine-grained: kernel stores reach it)
+  uint32_t* dflags;         // its device address
+  hipEvent_t run_ev[2];     // completion of run k in run_ev[k & 1]
+  long long runs;           // runs launched in learned mode
+  int mode_init[2];         // modes of runs 0 and 1
+  int mode_force;           // -1: automatic; 0 / 1: every run plain / learned
+  // neutral-H source planes (FP64 only): device copies owned by the
+  // simulation, null when not set
+  ast_sources_t src;
 };
 typedef struct ast_sim ast_sim_t;
 
@@ -66,7 +81,14 @@ static size_t elem(const ast_sim_t* s) { return s->precision ? sizeof(float) : s
 // steps per captured graph (even: a replay ends on the plane it starts from)
 constexpr int kChunk = 20;
 
-static ast::StepTime step_time(ast_sim_t* S, int j) {
+// the learned mode applies: FP32 exact solver with a learned guess and no
+// face window (the learned kernels need full waves)
+static bool learned_mode(const ast_sim_t* S) {
+  return S->precision == 1 && S->method == AST_EXACT && S->mlp != nullptr &&
+         S->g.ho_lo == -INFINITY && S->g.ho_hi == INFINITY;
+}
+
+static ast::StepTime step_time(ast_sim_t* S, int j, int mode) {
   double* T = S->scal;
   ast::StepTime tm;
   tm.nslots = ast::kMaxSlots;
@@ -74,14 +96,16 @@ static ast::StepTime step_time(ast_sim_t* S, int j) {
   tm.cur = ast::slot_set(T, j);
   tm.rst = ast::slot_set(T, j + 1);
   tm.T = T;
-  tm.gate = reinterpret_cast<uint32_t*>(T + ast::kGateWord);
+  tm.strong = mode >= 0 ? reinterpret_cast<uint32_t*>(T + ast::kGateWord) : nullptr;
+  tm.learned = mode > 0;
   return tm;
 }
 
 // n steps from plane `from`.  FP32: step j lowers slot set j (mod 3), takes
 // its dt from set j-1 and advances the time; a fold of the last set closes
-// the run, so between runs the time block reads as after funk_time.
-static int launch_steps(ast_sim_t* S, int from, int n) {
+// the run, so between runs the time block reads as after funk_time.  mode:
+// the learned mode of the run (-1: none).
+static int launch_steps(ast_sim_t* S, int from, int n, int mode) {
   double* T = S->scal;
   for (int j = 0; j < n; j++) {
     const int f = from ^ (j & 1);
@@ -92,18 +116,21 @@ static int launch_steps(ast_sim_t* S, int from, int n) {
     int rc;
     if (S->precision == 0) {
       rc = ast_step_f64_rows(&S->g, (const double*)s, (const double*)u, (double*)s2,
-                             (double*)u2, T, T + 1, S->method, S->collect ? S->stats : nullptr,
-                             S->rows64, S->stream);
+                             (double*)u2, T, T + 1, T + 2, S->method,
+                             S->collect ? S->stats : nullptr,
+                             S->rows64, S->src.nn2 || S->src.nn3 ? &S->src : nullptr, S->stream);
       if (rc == AST_OK) rc = ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
     } else {
-      const ast::StepTime tm = step_time(S, j);
+      const ast::StepTime tm = step_time(S, j, mode);
       rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
                              T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
                              &tm, S->stream);
     }
     if (rc != AST_OK) return rc;
   }
-  if (S->precision == 1 && n > 0) return ast_funk_time_fold(T, n - 1, ast::kMaxSlots, S->stream);
+  if (S->precision == 1 && n > 0)
+    return ast_funk_time_fold(T, n - 1, ast::kMaxSlots, mode >= 0 ? S->dflags : nullptr,
+                              S->stream);
   return AST_OK;
 }
 
@@ -117,11 +144,13 @@ static int clear_slot_sets(ast_sim_t* S) {
 }
 
 static void drop_graph(ast_sim_t* S) {
-  if (S->exec) (void)hipGraphExecDestroy(S->exec);
-  if (S->graph) (void)hipGraphDestroy(S->graph);
-  S->exec = nullptr;
-  S->graph = nullptr;
-  S->graph_ok = false;
+  for (int m = 0; m < 2; m++) {
+    if (S->exec[m]) (void)hipGraphExecDestroy(S->exec[m]);
+    if (S->graph[m]) (void)hipGraphDestroy(S->graph[m]);
+    S->exec[m] = nullptr;
+    S->graph[m] = nullptr;
+    S->graph_ok[m] = false;
+  }
 }
 
 extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, ast_sim_t** out) {
@@ -133,6 +162,10 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
     return ast_fail(AST_ERR_ARG, "ast_sim_create: precision %d (0 = FP64, 1 = FP32)", precision);
   if (method < AST_HLL || method > AST_HLLC_A)
     return ast_fail(AST_ERR_ARG, "ast_sim_create: method %d", method);
+  for (int k = 0; k < 4 && precision == 1; k++)
+    if (g->bc[k] == AST_BC_DIRICHLET_PULSED)
+      return ast_fail(AST_ERR_ARG, "ast_sim_create: the pulsed inflow of side %d is served by the "
+                      "FP64 step only", k);
   if (ast_device_count() < 1) return ast_fail(AST_ERR_NOGPU, "ast_sim_create: no HIP device");
   ast_sim_t* S = new (std::nothrow) ast_sim_t();
   if (!S) return ast_fail(AST_ERR_SIZE, "ast_sim_create: out of host memory");
@@ -147,14 +180,24 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
   if (e == hipSuccess) e = hipMalloc((void**)&S->scal, ast::kTimeBlockDoubles * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&S->stats, sizeof(ast_stats_t));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&S->hflags, 2 * sizeof(uint32_t),
+                                           hipHostMallocMapped | hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&S->dflags, S->hflags, 0);
+  for (int k = 0; k < 2 && e == hipSuccess; k++)
+    e = hipEventCreateWithFlags(&S->run_ev[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     ast_sim_t* tmp = S;
     for (int i = 0; i < 4; i++) if (tmp->plane[i]) (void)hipFree(tmp->plane[i]);
     if (tmp->scal) (void)hipFree(tmp->scal);
     if (tmp->stats) (void)hipFree(tmp->stats);
+    if (tmp->hflags) (void)hipHostFree(tmp->hflags);
+    for (int k = 0; k < 2; k++) if (tmp->run_ev[k]) (void)hipEventDestroy(tmp->run_ev[k]);
+    if (tmp->stream) (void)hipStreamDestroy(tmp->stream);
     delete tmp;
     return ast_hip_fail(e, "ast_sim_create: allocation");
   }
+  S->hflags[0] = S->hflags[1] = 0u;
+  S->mode_force = -1;
   // kernel.cu:3321-3324: T = 1e7, T_do = 1e-9, TT = 0, i = 0; dt slots 1e7
   double init[ast::kSlot0] = {1e7, 1e-9};
   CK(hipMemcpy(S->scal, init, sizeof init, hipMemcpyHostToDevice));
@@ -171,8 +214,40 @@ extern "C" int ast_sim_destroy(ast_sim_t* S) {
   for (int i = 0; i < 4; i++) (void)hipFree(S->plane[i]);
   (void)hipFree(S->scal);
   (void)hipFree(S->stats);
+  (void)hipHostFree(S->hflags);
+  (void)hipFree(const_cast<double*>(S->src.nn2));
+  (void)hipFree(const_cast<double*>(S->src.nn3));
+  for (int k = 0; k < 2; k++) (void)hipEventDestroy(S->run_ev[k]);
   (void)hipStreamDestroy(S->stream);
   delete S;
+  return AST_OK;
+}
+
+// Neutral-H source planes from host memory (the reference fills nn1..nn3 on
+// the host and copies them once, kernel.cu:3337-3339, 3453-3455): nn2 N*M
+// triples, nn3 N*M doubles, either may be NULL; both NULL removes them.
+extern "C" int ast_sim_set_sources(ast_sim_t* S, const double* nn2, const double* nn3,
+                                   double kmom, double ken) {
+  if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_set_sources: null sim");
+  if (S->precision != 0 && (nn2 || nn3))
+    return ast_fail(AST_ERR_ARG, "ast_sim_set_sources: source planes need an FP64 simulation");
+  CK(hipStreamSynchronize(S->stream));
+  drop_graph(S);  // captured runs hold the old planes
+  CK(hipFree(const_cast<double*>(S->src.nn2)));
+  CK(hipFree(const_cast<double*>(S->src.nn3)));
+  S->src = ast_sources_t{nullptr, nullptr, kmom, ken};
+  if (nn2) {
+    double* d = nullptr;
+    CK(hipMalloc(&d, S->cells * 3 * sizeof(double)));
+    S->src.nn2 = d;
+    CK(hipMemcpy(d, nn2, S->cells * 3 * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (nn3) {
+    double* d = nullptr;
+    CK(hipMalloc(&d, S->cells * sizeof(double)));
+    S->src.nn3 = d;
+    CK(hipMemcpy(d, nn3, S->cells * sizeof(double), hipMemcpyHostToDevice));
+  }
   return AST_OK;
 }
 
@@ -330,7 +405,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
     ncand = nd;
   }
   if (reps < 1) reps = 10;
-  double saved[4];
+  double saved[ast::kGateWord + 1];  // T, T_do, TT, i and the learned-mode words
   CK(hipStreamSynchronize(S->stream));
   CK(hipMemcpy(saved, S->scal, sizeof saved, hipMemcpyDeviceToHost));
   hipEvent_t e0, e1;
@@ -343,16 +418,19 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   void* u2 = S->plane[2 * (1 - from) + 1];
   int best = S->precision == 1 ? S->krows : S->rows64;
   float best_t = 1e30f;
-  // the first step of a run: dt from T_do, the learned mode from its word
-  const ast::StepTime tm = step_time(S, 0);
+  // the first step of a run (dt from T_do), in the kernel the runs take: with
+  // the learned start attached, the plain solver that detects strong faces
+  // unless the learned start is forced on
+  const ast::StepTime tm = step_time(S, 0, learned_mode(S) ? (S->mode_force == 1 ? 1 : 0) : -1);
   auto step = [&](int kr) {
     return S->precision == 1
                ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                    (float*)u2, S->scal, S->scal + 1, S->method, S->mlp, nullptr,
                                    kr, &tm, S->stream)
                : ast_step_f64_rows(&S->g, (const double*)sp, (const double*)up, (double*)s2,
-                                   (double*)u2, S->scal, S->scal + 1, S->method, nullptr, kr,
-                                   S->stream);
+                                   (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
+                                   nullptr, kr,
+                                   S->src.nn2 || S->src.nn3 ? &S->src : nullptr, S->stream);
   };
   auto valid = [&](int kr) {
     if (S->precision == 0) return kr >= 1 && kr <= 4096;
@@ -414,10 +492,15 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   return AST_OK;
 }
 
-static int build_graph(ast_sim_t* S) {
-  drop_graph(S);
+static int build_graph(ast_sim_t* S, int mode) {
+  const int m = mode > 0 ? 1 : 0;
+  if (S->exec[m]) (void)hipGraphExecDestroy(S->exec[m]);
+  if (S->graph[m]) (void)hipGraphDestroy(S->graph[m]);
+  S->exec[m] = nullptr;
+  S->graph[m] = nullptr;
+  S->graph_ok[m] = false;
   CK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-  int rc = launch_steps(S, 0, kChunk);
+  int rc = launch_steps(S, 0, kChunk, mode);
   hipGraph_t gr = nullptr;
   hipError_t e = hipStreamEndCapture(S->stream, &gr);
   if (rc != AST_OK) {
@@ -425,35 +508,94 @@ static int build_graph(ast_sim_t* S) {
     return rc;
   }
   if (e != hipSuccess) return ast_hip_fail(e, "ast_sim_run: hipStreamEndCapture");
-  S->graph = gr;
-  CK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
-  S->graph_ok = true;
+  S->graph[m] = gr;
+  CK(hipGraphInstantiate(&S->exec[m], S->graph[m], nullptr, nullptr, 0));
+  S->graph_ok[m] = true;
+  return AST_OK;
+}
+
+// The learned mode of the next run (learned_mode): its own initial value
+// for the first two runs, then the strong-face flag of the run two before,
+// read after that run's completion event (the run in between stays queued).
+static int next_run_mode(ast_sim_t* S, int* mode) {
+  if (S->mode_force >= 0) {
+    *mode = S->mode_force;
+    return AST_OK;
+  }
+  const long long k = S->runs;
+  if (k < 2) {
+    *mode = S->mode_init[k];
+    return AST_OK;
+  }
+  CK(hipEventSynchronize(S->run_ev[k & 1]));
+  *mode = __atomic_load_n(&S->hflags[k & 1], __ATOMIC_ACQUIRE) ? 1 : 0;
+  return AST_OK;
+}
+
+// One run of n steps from plane `from` (graph: a replay of the captured
+// kChunk steps of its mode, from plane 0).
+static int one_run(ast_sim_t* S, int from, int n, bool graph) {
+  const bool lm = learned_mode(S);
+  int mode = -1, rc;
+  if (lm && (rc = next_run_mode(S, &mode)) != AST_OK) return rc;
+  if (graph) {
+    const int m = mode > 0 ? 1 : 0;
+    if (!S->graph_ok[m] && (rc = build_graph(S, mode)) != AST_OK) return rc;
+    CK(hipGraphLaunch(S->exec[m], S->stream));
+  } else if ((rc = launch_steps(S, from, n, mode)) != AST_OK) {
+    return rc;
+  }
+  if (lm) {
+    CK(hipEventRecord(S->run_ev[S->runs & 1], S->stream));
+    S->runs++;
+  }
+  return AST_OK;
+}
+
+// The learned modes of the next two runs (waits for every launched run).
+extern "C" int ast_sim_learned_modes(ast_sim_t* S, int* modes) {
+  if (!S || !modes) return ast_fail(AST_ERR_ARG, "ast_sim_learned_modes: null argument");
+  CK(hipStreamSynchronize(S->stream));
+  const long long k = S->runs;
+  for (int d = 0; d < 2; d++)
+    modes[d] = S->mode_force >= 0
+                   ? S->mode_force
+                   : (k + d < 2 ? S->mode_init[k + d] : (S->hflags[(k + d) & 1] ? 1 : 0));
+  return AST_OK;
+}
+
+extern "C" int ast_sim_set_learned_mode(ast_sim_t* S, int mode) {
+  if (!S || mode < -1 || mode > 1)
+    return ast_fail(AST_ERR_ARG, "ast_sim_set_learned_mode: mode %d (-1, 0 or 1)", mode);
+  S->mode_force = mode;
   return AST_OK;
 }
 
 // Advances `steps` steps asynchronously on the simulation stream.  With
 // use_graph, runs of kChunk steps replay the captured graph (which starts
 // from plane 0, so an odd position first takes one eager step); the rest
-// are launched directly.  Both give the same result bit for bit.
+// are launched directly.  Both give the same result bit for bit.  In the
+// learned mode the eager path runs the same runs (1 step from an odd
+// position, then kChunk steps at a time, then the rest), so that each run
+// takes the same mode in both.
 extern "C" int ast_sim_run(ast_sim_t* S, int steps, int use_graph) {
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_run: null sim");
   if (steps < 0) return ast_fail(AST_ERR_ARG, "ast_sim_run: steps %d", steps);
   int rc;
-  if (!use_graph) {
-    if ((rc = launch_steps(S, S->cur, steps)) != AST_OK) return rc;
+  if (!use_graph && !learned_mode(S)) {
+    if ((rc = launch_steps(S, S->cur, steps, -1)) != AST_OK) return rc;
     S->cur ^= steps & 1;
     return AST_OK;
   }
   if (S->cur == 1 && steps > 0) {
-    if ((rc = launch_steps(S, 1, 1)) != AST_OK) return rc;
+    if ((rc = one_run(S, 1, 1, false)) != AST_OK) return rc;
     S->cur = 0;
     steps--;
   }
-  if (steps >= kChunk && !S->graph_ok)
-    if ((rc = build_graph(S)) != AST_OK) return rc;
-  for (; steps >= kChunk; steps -= kChunk) CK(hipGraphLaunch(S->exec, S->stream));
+  for (; steps >= kChunk; steps -= kChunk)
+    if ((rc = one_run(S, 0, kChunk, use_graph != 0)) != AST_OK) return rc;
   if (steps > 0) {
-    if ((rc = launch_steps(S, 0, steps)) != AST_OK) return rc;
+    if ((rc = one_run(S, 0, steps, false)) != AST_OK) return rc;
     S->cur = steps & 1;
   }
   return AST_OK;
@@ -581,7 +723,7 @@ extern "C" int ast_sim_read_b31(ast_sim_t* S, const char* path) {
 struct CkptHeader {
   char magic[8];
   int32_t version, N, M, precision, i;
-  uint32_t gate_mode;  // learned mode of the next step (time_slots.h)
+  uint32_t gate_mode;  // learned modes of the next two runs (bits 0 and 1)
   double T, T_do, TT;
 };
 
@@ -596,7 +738,13 @@ extern "C" int ast_sim_save(ast_sim_t* S, const char* path) {
   h.precision = S->precision;
   int rc = ast_sim_get_time(S, &h.T, &h.T_do, &h.TT, &h.i);
   if (rc != AST_OK) return rc;
-  CK(hipMemcpy(&h.gate_mode, S->scal + ast::kGateWord, sizeof h.gate_mode, hipMemcpyDeviceToHost));
+  // every launched run has completed (ast_sim_get_time synchronised): the
+  // modes of the next two runs are known
+  {
+    int m[2];
+    if ((rc = ast_sim_learned_modes(S, m)) != AST_OK) return rc;
+    h.gate_mode = (uint32_t)(m[0] | (m[1] << 1));
+  }
   const size_t bytes = S->cells * 2 * elem(S);
   std::vector<char> a(bytes), b(bytes);
   CK(hipMemcpy(a.data(), S->plane[2 * S->cur], bytes, hipMemcpyDeviceToHost));
@@ -636,7 +784,11 @@ extern "C" int ast_sim_load(ast_sim_t* S, const char* path) {
   S->cur = 0;
   CK(hipMemcpy(S->plane[0], a.data(), bytes, hipMemcpyHostToDevice));
   CK(hipMemcpy(S->plane[1], b.data(), bytes, hipMemcpyHostToDevice));
-  const uint32_t gate[2] = {h.gate_mode ? 1u : 0u, 0u};
-  CK(hipMemcpy(S->scal + ast::kGateWord, gate, sizeof gate, hipMemcpyHostToDevice));
+  const uint32_t words[2] = {0u, 0u};  // {strong, run}
+  CK(hipMemcpy(S->scal + ast::kGateWord, words, sizeof words, hipMemcpyHostToDevice));
+  S->mode_init[0] = (int)(h.gate_mode & 1u);
+  S->mode_init[1] = (int)((h.gate_mode >> 1) & 1u);
+  S->runs = 0;
+  S->hflags[0] = S->hflags[1] = 0u;
   return ast_sim_set_time(S, h.T, h.T_do, h.TT, h.i);
 }

@@ -53,10 +53,9 @@ extern "C" int ast_get_tile_rows(void) { return g_krows; }
 // always on).  Otherwise the step lowers the slots of tm->cur and, with
 // tm->prev, takes its dt from the previous step's slots and advances the
 // time itself.  With the exact method, a learned guess, no window and
-// tm->gate, the step is two gated launches: the plain solver (mode 0), which
-// flags strong two-shock faces in the strong word of tm->cur, and the solver
-// with the learned start (mode 1); the mode is the strong flag of tm->prev,
-// or tm->gate[0] without one.
+// tm->strong, the runtime has chosen the mode of the step: the solver with
+// the learned start (tm->learned) or the plain solver; both flag strong
+// two-shock faces in *tm->strong.
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
                       ast_stats_t* stats, int krows, const StepTime* tm, void* stream) {
@@ -90,16 +89,9 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
     case AST_HLLC: return step_launch<AST_HLLC>(a);
     case AST_HLLC_A: return step_launch<AST_HLLC_A>(a);
     case AST_EXACT:  // the learned start only where it can run (no window: full waves)
-      if (mlp != nullptr && !window && tm != nullptr && tm->gate != nullptr) {
-        StepArgs a0 = a;
-        a0.ex.gate = tm->prev ? reinterpret_cast<const uint32_t*>(tm->prev + 1) : tm->gate;
-        a0.ex.strong = reinterpret_cast<uint32_t*>(tm->cur + 1);
-        StepArgs a1 = a0;
-        a0.ex.gate_mode = 0;
-        a1.ex.gate_mode = 1;
-        rc = step_launch<kExactDetect>(a0);
-        if (rc != AST_OK) return rc;
-        return step_launch<kExactLearned>(a1);
+      if (mlp != nullptr && !window && tm != nullptr && tm->strong != nullptr) {
+        a.ex.strong = tm->strong;
+        return tm->learned ? step_launch<kExactLearned>(a) : step_launch<kExactDetect>(a);
       }
       if (mlp != nullptr && !window) return step_launch<kExactLearned>(a);
       return step_launch<AST_EXACT>(a);

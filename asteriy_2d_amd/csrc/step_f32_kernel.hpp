@@ -56,15 +56,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   int nwg) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
-  // gated launch: this kernel's mode is not the selected one (grid-uniform).
-  // The learned kernel, idle unless the previous step had strong faces, tests
-  // its gate first; the others read the gate word before the dt slots (it
-  // lies beside them), so the two loads overlap, and test it before anything
-  // is written
-  if (METHOD == kExactLearned && ex.gate != nullptr && *ex.gate != ex.gate_mode) return;
-  const uint32_t gate = ex.gate != nullptr ? *ex.gate : ex.gate_mode;
   const double tdo = step_dt_load(T_do, kt);
-  if (gate != ex.gate_mode) return;
   step_dt_commit(T, kt, tdo);
   constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
   typename std::conditional<STATS, StatsF32,
@@ -74,20 +66,8 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   // wave index made explicitly uniform: rows and the loop stay scalar
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int band, m0, m1;
-  if constexpr (METHOD == kExactLearned) {
-    // several strip groups per workgroup when the grid is smaller than the
-    // work: the learned kernel runs on its co-resident capacity, so that it
-    // costs little in the steps in which it only reads the gate (the loop
-    // costs the other kernels registers, so they take one group each)
-    for (int item = blockIdx.x; item < nwg; item += gridDim.x) {  // workgroup-uniform
-      strip_rows(g.M, nbands, krows, xcd_tile(item, nwg), wave, band, m0, m1);
-      tmin = fmin(tmin, strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, band,
-                                                            m0, m1, st));
-    }
-  } else {
-    strip_rows(g.M, nbands, krows, xcd_tile(blockIdx.x, nwg), wave, band, m0, m1);
-    tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st);
-  }
+  strip_rows(g.M, nbands, krows, xcd_tile(blockIdx.x, nwg), wave, band, m0, m1);
+  tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st);
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
@@ -107,10 +87,7 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
     nwg = ((long long)nbands * -(long long)krows + WPB - 1) / WPB;
   }
   if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
-  // the gated learned kernel: 512 workgroups (two waves per SIMD, what its
-  // registers allow to be resident at once) that loop over the strip groups
-  const unsigned grid =
-      (unsigned)(METHOD == kExactLearned && ex.gate != nullptr && nwg > 512 ? 512 : nwg);
+  const unsigned grid = (unsigned)nwg;
   if (is_exact(METHOD) && stats != nullptr) {
     if constexpr (is_exact(METHOD))
       hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3(grid), dim3(NT), 0,

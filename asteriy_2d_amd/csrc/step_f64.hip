@@ -93,7 +93,7 @@ AST_DEV void face_values(const Q& c, const Q& dl, const Q& dr, Q& hi, Q& lo) {
 // nearest grid cell, so every lane holds a finite, positive-where-the-grid-is
 // state.
 AST_DEV Q load_q(const ast_grid_t& g, const double2* __restrict__ s,
-                 const double2* __restrict__ u, int n, int m) {
+                 const double2* __restrict__ u, int n, int m, double pulse) {
   const bool xin = (n >= 0) & (n < g.N), yin = (m >= 0) & (m < g.M);
   double2 a, b;
   if (xin & yin) {
@@ -101,7 +101,7 @@ AST_DEV Q load_q(const ast_grid_t& g, const double2* __restrict__ s,
     a = s[k];
     b = u[k];
   } else if ((xin | yin) & (n >= -2) & (n <= g.N + 1) & (m >= -2) & (m <= g.M + 1)) {
-    fetch_cell(g, s, u, n, m, a, b);
+    fetch_cell(g, s, u, n, m, a, b, pulse);
   } else {
     const int cn = n < 0 ? 0 : (n >= g.N ? g.N - 1 : n);
     const int cm = m < 0 ? 0 : (m >= g.M ? g.M - 1 : m);
@@ -174,7 +174,13 @@ struct StepArgs {
   double2* u2;
   double* T;
   const double* T_do;
+  const double* TT;  // simulation time (read only with a pulsed side; else null)
   ast_stats_t* stats;
+  // neutral-H source planes of add_MK (kernel.cu:2495-2498), null when absent:
+  // nn2 3 doubles per cell (x, y used), nn3 one
+  const double* nn2;
+  const double* nn3;
+  double kmom, ken;
   int rows;    // rows per strip
   int nbands;  // bands of WOUT columns
   int nwg;     // workgroups
@@ -199,6 +205,8 @@ __global__ __launch_bounds__(NT) void strip_kernel(StepArgs A) {
   const bool ho_x = high_order(g, x - g.dx / 2.0);  // west face (kernel.cu:2843)
   const bool ho_y = high_order(g, x);               // y faces: the cell x (kernel.cu:2816)
   const double T_do = *A.T_do;
+  // pulsed Dirichlet sides: the density factor at the time before the step
+  const double pulse = A.TT != nullptr ? pulse_factor(g, *A.TT) : 1.0;
   const double kx = T_do / g.dx, ky = T_do / g.dy;
   const bool frz = g.frozen_ax > 0.0;
   // frozen grid cells are copied (kernel.cu:2317-2324); a face votes for dt
@@ -215,11 +223,11 @@ __global__ __launch_bounds__(NT) void strip_kernel(StepArgs A) {
 
   if (m0 < g.M && band * WOUT < g.N) {  // wave-uniform
     // prologue: rows m0-2 .. m0+2; the south face of row m0
-    const Q qa = load_q(g, A.s, A.u, n, m0 - 2);
-    const Q qb = load_q(g, A.s, A.u, n, m0 - 1);
-    Q q0 = load_q(g, A.s, A.u, n, m0);
-    Q q1 = load_q(g, A.s, A.u, n, m0 + 1);
-    Q q2 = load_q(g, A.s, A.u, n, m0 + 2);
+    const Q qa = load_q(g, A.s, A.u, n, m0 - 2, pulse);
+    const Q qb = load_q(g, A.s, A.u, n, m0 - 1, pulse);
+    Q q0 = load_q(g, A.s, A.u, n, m0, pulse);
+    Q q1 = load_q(g, A.s, A.u, n, m0 + 1, pulse);
+    Q q2 = load_q(g, A.s, A.u, n, m0 + 2, pulse);
     Q d1 = q_sub(q1, q0);  // q(m+1) - q(m)
     Q vn;                  // north face value of row m
     FluxD Gs;              // flux through the south face of row m
@@ -236,7 +244,7 @@ __global__ __launch_bounds__(NT) void strip_kernel(StepArgs A) {
     }
     bool fr_m = frozen(n, m0);
     for (int m = m0; m < m1; m++) {
-      const Q q3 = load_q(g, A.s, A.u, n, m + 3);  // for the next row
+      const Q q3 = load_q(g, A.s, A.u, n, m + 3, pulse);  // for the next row
       // y: face values of row m+1; the north face of row m
       const Q d2 = q_sub(q2, q1);
       Q vn1, vs1;
@@ -279,6 +287,17 @@ __global__ __launch_bounds__(NT) void strip_kernel(StepArgs A) {
         dmx -= tr * q0.vx;
         dmy -= tr * q0.vy;
         dE -= ty * q0.vy * (gas.g * gas.ig1 * q0.p + ke);
+      }
+      if (A.nn2 != nullptr || A.nn3 != nullptr) {  // wave-uniform
+        // neutral-H sources (kernel.cu:2495-2498): momentum T_do (rho kmom)
+        // nn2.{x,y}, energy T_do ken nn3; output cells only (in bounds)
+        const size_t k = out_col ? (size_t)m * g.N + n : 0;
+        if (A.nn2 != nullptr) {
+          const double tk = T_do * (q0.r * A.kmom);
+          dmx += tk * (out_col ? A.nn2[3 * k] : 0.0);
+          dmy += tk * (out_col ? A.nn2[3 * k + 1] : 0.0);
+        }
+        if (A.nn3 != nullptr) dE += (T_do * A.ken) * (out_col ? A.nn3[k] : 0.0);
       }
       // density floor (kernel.cu:3060-3064): a non-positive density keeps
       // the old one, and the kinetic energy of the new state uses it
@@ -349,10 +368,11 @@ static int launch(const StepArgs& a, bool stats, bool window, hipStream_t st) {
 }  // namespace s64
 }  // namespace ast
 
-// The step with a given strip height (rows > 0; 0 = the default).
+// The step with a given strip height (rows > 0; 0 = the default) and
+// optional neutral-H source planes (src may be null).
 int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, double* s2,
-                      double* u2, double* T, const double* T_do, int method, ast_stats_t* stats,
-                      int rows, void* stream) {
+                      double* u2, double* T, const double* T_do, const double* TT, int method,
+                      ast_stats_t* stats, int rows, const ast_sources_t* src, void* stream) {
   using namespace ast::s64;
   int rc = ast_grid_check(g);
   if (rc != AST_OK) return rc;
@@ -371,7 +391,18 @@ int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, dou
   a.u2 = (double2*)u2;
   a.T = T;
   a.T_do = T_do;
+  bool pulsed = false;
+  for (int k = 0; k < 4; k++) pulsed |= g->bc[k] == AST_BC_DIRICHLET_PULSED;
+  if (pulsed && !TT)
+    return ast_fail(AST_ERR_ARG, "ast_step_f64: a pulsed Dirichlet side needs the time TT");
+  a.TT = pulsed ? TT : nullptr;
   a.stats = stats;
+  a.nn2 = src ? src->nn2 : nullptr;
+  a.nn3 = src ? src->nn3 : nullptr;
+  a.kmom = src ? src->kmom : 0.0;
+  a.ken = src ? src->ken : 0.0;
+  if (a.nn2 == s || a.nn2 == u || a.nn3 == s || a.nn3 == u)
+    return ast_fail(AST_ERR_ARG, "ast_step_f64: a source plane aliases a state plane");
   a.rows = rows;
   a.nbands = (g->N + WOUT - 1) / WOUT;
   const long long strips = (g->M + rows - 1) / rows;
@@ -396,6 +427,14 @@ int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, dou
 extern "C" int ast_step_f64(const ast_grid_t* g, const double* s, const double* u, double* s2,
                             double* u2, double* T, const double* T_do, double* TT, int method,
                             ast_stats_t* stats, void* stream) {
-  (void)TT;  // accepted for signature parity; unused, as in the reference's live body
-  return ast_step_f64_rows(g, s, u, s2, u2, T, T_do, method, stats, 0, stream);
+  // TT is read only with a pulsed Dirichlet side (unused by the reference's
+  // live body)
+  return ast_step_f64_rows(g, s, u, s2, u2, T, T_do, TT, method, stats, 0, nullptr, stream);
+}
+
+extern "C" int ast_step_f64_src(const ast_grid_t* g, const double* s, const double* u,
+                                double* s2, double* u2, double* T, const double* T_do, double* TT,
+                                int method, const ast_sources_t* src, ast_stats_t* stats,
+                                void* stream) {
+  return ast_step_f64_rows(g, s, u, s2, u2, T, T_do, TT, method, stats, 0, src, stream);
 }

@@ -162,9 +162,13 @@ static inline bool strip_fits(const ast_grid_t& g, int krows) {
 // positive densities and pressures in every cell and ghost: a Dirichlet side
 // must hold a positive state.
 static inline int check_positive_bc(const ast_grid_t& g) {
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < 4; k++) {
     if (g.bc[k] == AST_BC_DIRICHLET && !(g.bc_state[k][0] > 0.0 && g.bc_state[k][1] > 0.0))
       return ast_fail(AST_ERR_ARG, "FP32 step: Dirichlet side %d needs rho > 0 and p > 0", k);
+    if (g.bc[k] == AST_BC_DIRICHLET_PULSED)
+      return ast_fail(AST_ERR_ARG, "FP32 step: the pulsed inflow of side %d is served by the "
+                      "FP64 step only", k);
+  }
   return AST_OK;
 }
 

@@ -17,9 +17,10 @@
 #include <stdint.h>
 
 namespace ast {
-// word 4 holds two 32-bit words {mode, strong} of the learned mode: the
-// gated exact-solver launches run the plain solver in mode 0 and the one
-// with the learned newton start in mode 1
+// word 4 holds two 32-bit words {strong, run}: the exact-solver kernels of a
+// run OR 1 into `strong` when they meet a strong two-shock face, and the
+// kernel that closes the run hands `strong` to the host (the learned mode
+// of a later run), clears it and counts the run in `run`
 constexpr int kGateWord = 4;
 constexpr int kSlot0 = 32;
 constexpr int kSlotStride = 32;
@@ -35,13 +36,14 @@ inline double* slot_set(double* T, int j) { return T + kSlot0 + kSetDoubles * (j
 // step lowers *T with one atomic per workgroup and reads dt from *T_do).
 struct StepTime {
   int nslots;          // slots of a set in use (1..kMaxSlots)
-  const double* prev;  // set of the previous step: dt = min of its slots and,
-                       // with the learned gate, mode = its strong flag; null:
-                       // dt = *T_do and mode = gate[0] (first step of a run)
+  const double* prev;  // set of the previous step: dt = min of its slots;
+                       // null: dt = *T_do (first step of a run)
   double* cur;         // set this step's workgroups lower (and flag strong in)
   double* rst;         // set cleared by this step (may be null)
   double* T;           // time block: with prev, workgroup 0 writes T_do, TT, i
-                       // and the mode word as funk_time would have
-  uint32_t* gate;      // {mode, strong} words (null: no learned gate)
+                       // as funk_time would have
+  uint32_t* strong;    // run-level strong-face flag (null: not detected)
+  int learned;         // exact solver with a learned guess: 1 the learned
+                       // start, 0 the plain solver (which detects strong faces)
 };
 }  // namespace ast

@@ -13,10 +13,14 @@ from dataclasses import dataclass, field
 from typing import List, Sequence
 
 HLL, HLLC, EXACT, HLLC_A = 0, 1, 2, 3
+HLLCQ, HLLCQ_A = 4, 5  # passive-scalar fluxes (ast_flux_q_*)
 METHODS = {"hll": HLL, "hllc": HLLC, "exact": EXACT, "hllc_a": HLLC_A}
 PLANAR, AXISYMMETRIC = 0, 1
-BC_OUTFLOW, BC_REFLECT, BC_DIRICHLET, BC_OUTFLOW_CLAMPED = 0, 1, 2, 3
+BC_OUTFLOW, BC_REFLECT, BC_DIRICHLET, BC_OUTFLOW_CLAMPED, BC_DIRICHLET_PULSED = 0, 1, 2, 3, 4
 XLO, XHI, YLO, YHI = 0, 1, 2, 3
+# the inflow perturbation of add2 (kernel.cu:2022-2025): 1% density, period 1,
+# rows y < 0.5, until TT = 3
+PULSE_ADD2 = (0.01, 2.0 * math.pi, 0.5, 3.0)
 ITER_CAP = 1100
 
 
@@ -41,6 +45,7 @@ class CGrid(ctypes.Structure):
         ("frozen_x0", ctypes.c_double),
         ("frozen_ax", ctypes.c_double),
         ("frozen_ay", ctypes.c_double),
+        ("bc_pulse", ctypes.c_double * 4),
     ]
 
 
@@ -78,6 +83,10 @@ class Grid:
     frozen_x0: float = 0.0
     frozen_ax: float = 0.0
     frozen_ay: float = 0.0
+    # BC_DIRICHLET_PULSED sides: density factor 1 + a sin(w TT) for rows with
+    # y < y_lim while TT < t_lim (a, w, y_lim, t_lim); add2's inflow
+    # perturbation (kernel.cu:2022-2025) is PULSE_ADD2
+    bc_pulse: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
 
     @classmethod
     def from_extent(cls, N: int, M: int, x_min: float, x_max: float,
@@ -109,6 +118,8 @@ class Grid:
         g.gamma, g.krit = self.gamma, self.krit
         g.ho_lo, g.ho_hi = self.ho_lo, self.ho_hi
         g.frozen_x0, g.frozen_ax, g.frozen_ay = self.frozen_x0, self.frozen_ax, self.frozen_ay
+        for i in range(4):
+            g.bc_pulse[i] = float(self.bc_pulse[i])
         return g
 
     def replace(self, **kw) -> "Grid":

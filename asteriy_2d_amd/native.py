@@ -18,6 +18,13 @@ _c_int, _c_size, _c_double, _c_float, _vp = (ctypes.c_int, ctypes.c_size_t, ctyp
                                              ctypes.c_float, ctypes.c_void_p)
 _gp = ctypes.POINTER(CGrid)
 
+
+class CSources(ctypes.Structure):
+    """ast_sources_t: device pointers of the neutral-H planes and their
+    coefficients."""
+    _fields_ = [("nn2", ctypes.c_void_p), ("nn3", ctypes.c_void_p),
+                ("kmom", ctypes.c_double), ("ken", ctypes.c_double)]
+
 # name -> (restype, argtypes); every symbol declared in include/asteriy.h
 SIGNATURES = {
     "ast_last_error": (ctypes.c_char_p, []),
@@ -28,7 +35,11 @@ SIGNATURES = {
                               _c_double, _vp, _vp, _vp]),
     "ast_flux_f32": (_c_int, [_c_int, _c_size, _vp, _c_float, _c_float, _c_float, _c_float,
                               _c_float, _vp, _vp, _vp]),
+    "ast_flux_q_f64": (_c_int, [_c_int, _c_size, _vp, _c_double, _c_double, _c_double,
+                                _c_double, _c_double, _vp, _vp]),
     "ast_step_f64": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp]),
+    "ast_step_f64_src": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
+                                  ctypes.POINTER(CSources), _vp, _vp]),
     "ast_step_f32": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp]),
     "ast_funk_time": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ast_conservation_work_doubles": (_c_size, []),

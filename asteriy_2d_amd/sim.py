@@ -28,6 +28,9 @@ SIM_SIGNATURES = {
     "ast_sim_get_stats": (_c_int, [_vp, ctypes.POINTER(CStats), _c_int]),
     "ast_sim_collect_stats": (_c_int, [_vp, _c_int]),
     "ast_sim_set_mlp": (_c_int, [_vp, _vp]),
+    "ast_sim_set_sources": (_c_int, [_vp, _vp, _vp, _c_double, _c_double]),
+    "ast_sim_learned_modes": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "ast_sim_set_learned_mode": (_c_int, [_vp, _c_int]),
     "ast_sim_set_tile_rows": (_c_int, [_vp, _c_int]),
     "ast_sim_stream": (_vp, [_vp]),
     "ast_sim_autotune": (_c_int, [_vp, ctypes.POINTER(_c_int), _c_int, _c_int,
@@ -124,6 +127,31 @@ class Simulation:
 
     def set_mlp(self, mlp):
         native.call("ast_sim_set_mlp", self.h, _vp(mlp.buffer.data_ptr()) if mlp else None)
+
+    def set_sources(self, nn2=None, nn3=None, kmom: float = 3.0 / 0.419231, ken: float = 0.0):
+        """Neutral-H source planes of add_MK (kernel.cu:2495-2498) for every
+        later step: nn2 (M, N, 3), nn3 (M, N) host float64 arrays (either may
+        be None); the defaults are the reference's n_H/Kn and 0.0*n_H/Kn
+        (Header.h:84-85).  FP64 simulations only."""
+        a2 = None if nn2 is None else np.ascontiguousarray(nn2, dtype=np.float64)
+        a3 = None if nn3 is None else np.ascontiguousarray(nn3, dtype=np.float64)
+        if a2 is not None and a2.shape != (self.grid.M, self.grid.N, 3):
+            raise ValueError(f"nn2: need shape {(self.grid.M, self.grid.N, 3)}, got {a2.shape}")
+        if a3 is not None and a3.shape != (self.grid.M, self.grid.N):
+            raise ValueError(f"nn3: need shape {(self.grid.M, self.grid.N)}, got {a3.shape}")
+        native.call("ast_sim_set_sources", self.h,
+                    None if a2 is None else a2.ctypes.data_as(_vp),
+                    None if a3 is None else a3.ctypes.data_as(_vp), float(kmom), float(ken))
+
+    def learned_modes(self):
+        """The learned modes (0 plain, 1 learned start) of the next two runs."""
+        m = (_c_int * 2)()
+        native.call("ast_sim_learned_modes", self.h, m)
+        return m[0], m[1]
+
+    def set_learned_mode(self, mode: int):
+        """-1 automatic (default), 0 plain solver, 1 learned start, every run."""
+        native.call("ast_sim_set_learned_mode", self.h, int(mode))
 
     def set_tile_rows(self, krows: int):
         native.call("ast_sim_set_tile_rows", self.h, int(krows))

@@ -8,6 +8,7 @@ The reference exposes (kernel.cu, Gas_Dinamic.cu):
   * the step kernel     Kernel_TVD(s, u, s2, u2, T, T_do, TT, method)
                                                      kernel.cu:2513
   * the time advance    funk_time(T, T_do, TT, i)    kernel.cu:1525
+  * the neutral-H source terms of add_MK (nn2, nn3)   kernel.cu:2293, 2495-2498
 This module offers the same names and argument meaning over torch device
 tensors, each call going through the C-ABI of libasteriy.so (HIP kernels for
 gfx950).  Tensors: s, u are (M, N, 2) float64 (FP64 path) or float32 (FP32
@@ -20,6 +21,7 @@ import ctypes
 
 from . import native
 from .grid import EXACT, HLL as M_HLL, HLLC as M_HLLC, HLLC_A as M_HLLC_A
+from .grid import HLLCQ as M_HLLCQ, HLLCQ_A as M_HLLCQ_A
 
 _STATS_WORDS = 10
 
@@ -50,13 +52,45 @@ def new_stats(device="cuda"):
     return torch.zeros(_STATS_WORDS, dtype=torch.int64, device=device)
 
 
+# the reference's neutral-H coupling constants n_H and Kn (Header.h:84-85):
+# add_MK's momentum coefficient is n_H/Kn, its energy coefficient 0.0*n_H/Kn
+N_H, KN = 3.0, 0.419231
+
+
+class NeutralSources:
+    """The neutral-H moment planes of add_MK (kernel.cu:2293, 2495-2498) on
+    the device: nn2 (M, N, 3) float64 (the reference's double3 plane; x and y
+    are read), nn3 (M, N) float64 or None; the momentum of a cell gains
+    T_do*(rho*kmom)*nn2.{x,y}, its energy T_do*ken*nn3."""
+
+    def __init__(self, nn2=None, nn3=None, kmom: float = N_H / KN, ken: float = 0.0 * N_H / KN):
+        import torch
+        for t, nm, last in ((nn2, "nn2", 3), (nn3, "nn3", None)):
+            if t is None:
+                continue
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                    and t.is_contiguous()):
+                raise TypeError(f"{nm} must be a contiguous float64 CUDA (HIP) tensor")
+            if last is not None and t.shape[-1] != last:
+                raise ValueError(f"{nm}: last dimension must be {last}")
+        self.nn2, self.nn3, self.kmom, self.ken = nn2, nn3, float(kmom), float(ken)
+
+    def c(self, cells: int):
+        for t, nm, per in ((self.nn2, "nn2", 3), (self.nn3, "nn3", 1)):
+            if t is not None and t.numel() != per * cells:
+                raise ValueError(f"{nm}: need {per * cells} values, got {t.numel()}")
+        return native.CSources(_ptr(self.nn2), _ptr(self.nn3), self.kmom, self.ken)
+
+
 def Kernel_TVD(grid, s, u, s2, u2, T, T_do, TT, method: int, stats=None, mlp=None,
-               stream=None):
+               stream=None, sources: NeutralSources | None = None):
     """One step of the reference's Kernel_TVD (kernel.cu:2513) on the GPU.
 
-    FP64 tensors run the bit-faithful cell-by-cell kernel (ast_step_f64);
-    FP32 tensors run the face-shared fast path (ast_step_f32), optionally
-    seeding the exact solver with a learned guess (`mlp`, see mlp.py)."""
+    FP64 tensors run the face-centric FP64 step (ast_step_f64, within 1e-12
+    of the reference's step), with `sources` the neutral-H terms of add_MK
+    (ast_step_f64_src); FP32 tensors run the fast path (ast_step_f32),
+    optionally seeding the exact solver with a learned guess (`mlp`, see
+    mlp.py)."""
     import torch
     cells = grid.N * grid.M
     dt = s.dtype
@@ -68,9 +102,17 @@ def Kernel_TVD(grid, s, u, s2, u2, T, T_do, TT, method: int, stats=None, mlp=Non
     if dt == torch.float64:
         if mlp is not None:
             raise ValueError("the learned guess is served by the FP32 step only")
-        native.call("ast_step_f64", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
-                    _ptr(T_do), _ptr(TT), int(method), _ptr(stats), _stream(stream))
+        if sources is not None:
+            cs = sources.c(cells)
+            native.call("ast_step_f64_src", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
+                        _ptr(T_do), _ptr(TT), int(method), ctypes.byref(cs), _ptr(stats),
+                        _stream(stream))
+        else:
+            native.call("ast_step_f64", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
+                        _ptr(T_do), _ptr(TT), int(method), _ptr(stats), _stream(stream))
     else:
+        if sources is not None:
+            raise ValueError("the neutral-H sources are served by the FP64 step only")
         native.call("ast_step_f32", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
                     _ptr(T_do), _ptr(TT), int(method),
                     _ptr(mlp.buffer) if mlp is not None else None, _ptr(stats),
@@ -125,6 +167,35 @@ def HLLC_Aleksashov(faces, n1, n2, rad, **kw):
     """kernel.cu:767 (zero magnetic field, as add_MK calls it) over a batch of
     faces."""
     return flux_batch(M_HLLC_A, faces, n1, n2, rad, **kw)
+
+
+def flux_q_batch(method: int, faces, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0, krit=0.05,
+                 out=None, stream=None):
+    """The passive-scalar operators over n faces: faces is a (10, n) float64
+    device tensor (the 8 planes of flux_batch, then Q_L, Q_R); returns (6, n):
+    the 5 planes of flux_batch, then the scalar's flux PQ.  method: HLLC (the
+    live solver's scalar), HLLCQ or HLLCQ_A."""
+    import torch
+    if (faces.dim() != 2 or faces.shape[0] != 10 or not faces.is_contiguous()
+            or faces.dtype != torch.float64):
+        raise ValueError("faces must be a contiguous (10, n) float64 tensor")
+    n = faces.shape[1]
+    if out is None:
+        out = torch.empty((6, n), dtype=faces.dtype, device=faces.device)
+    native.call("ast_flux_q_f64", int(method), n, _ptr(faces), float(n1), float(n2), float(rad),
+                float(gamma), float(krit), _ptr(out), _stream(stream))
+    return out
+
+
+def HLLCQ_Korolkov_2D(faces, n1, n2, rad, **kw):
+    """Gas_Dinamic.cu:249 over a batch of faces with a passive scalar."""
+    return flux_q_batch(M_HLLCQ, faces, n1, n2, rad, **kw)
+
+
+def HLLCQ_Aleksashov(faces, n1, n2, rad, **kw):
+    """Gas_Dinamic.cu:355 (zero magnetic field) over a batch of faces with a
+    passive scalar."""
+    return flux_q_batch(M_HLLCQ_A, faces, n1, n2, rad, **kw)
 
 
 def conservation_sums(grid, s, u, stream=None):

@@ -41,6 +41,10 @@ enum {
  *      AST_HLLC_A is HLLC_Aleksashov (kernel.cu:767-1079, add_MK's method 1)
  *      with the zero magnetic field it is called with. */
 enum { AST_HLL = 0, AST_HLLC = 1, AST_EXACT = 2, AST_HLLC_A = 3 };
+/* passive-scalar fluxes (ast_flux_q_*): HLLCQ_Korolkov_2D
+ * (Gas_Dinamic.cu:249) and HLLCQ_Aleksashov (Gas_Dinamic.cu:355, zero
+ * magnetic field) */
+enum { AST_HLLCQ = 4, AST_HLLCQ_A = 5 };
 
 /* ---- geometry of the conservative update ------------------------------- */
 enum {
@@ -54,8 +58,15 @@ enum {
   AST_BC_REFLECT = 1,          /* mirror, normal velocity negated,
                                   kernel.cu:2564-2569, 2609-2614            */
   AST_BC_DIRICHLET = 2,        /* fixed state, kernel.cu:2577-2580          */
-  AST_BC_OUTFLOW_CLAMPED = 3   /* zero gradient with the normal velocity
+  AST_BC_OUTFLOW_CLAMPED = 3,  /* zero gradient with the normal velocity
                                   limited to bc_vclamp, kernel.cu:2340-2354 */
+  AST_BC_DIRICHLET_PULSED = 4  /* fixed state whose density is multiplied by
+                                  1 + a*sin(w*TT) in the rows with y < y_lim
+                                  while TT < t_lim (bc_pulse = {a, w, y_lim,
+                                  t_lim}; TT the simulation time before the
+                                  step): the inflow perturbation of add2,
+                                  kernel.cu:2022-2025 (a = 0.01, w = 2 pi,
+                                  y_lim = 0.5, t_lim = 3).  FP64 step only */
 };
 enum { AST_XLO = 0, AST_XHI = 1, AST_YLO = 2, AST_YHI = 3 };
 
@@ -80,6 +91,7 @@ typedef struct ast_grid {
   double bc_vclamp[4];        /* AST_BC_OUTFLOW_CLAMPED normal-velocity cap */
   double ho_lo, ho_hi;        /* high-order face window on x                */
   double frozen_x0, frozen_ax, frozen_ay;
+  double bc_pulse[4];         /* AST_BC_DIRICHLET_PULSED: a, w, y_lim, t_lim */
 } ast_grid_t;
 
 /* Exact-solver and precision counters, accumulated on the device when a
@@ -130,6 +142,22 @@ int ast_flux_f64(int method, size_t n, const double* in, double n1, double n2,
 int ast_flux_f32(int method, size_t n, const float* in, float n1, float n2,
                  float rad, float gamma, float krit, float* out,
                  ast_stats_t* stats, void* stream);
+
+/* Passive-scalar fluxes, replacing
+ *   double HLLCQ_Korolkov_2D(const double2& Ls, const double2& Lu,
+ *       const double2& Rs, const double2& Ru, const double& LQ,
+ *       const double& RQ, double n1, double n2, double2& Ps, double2& Pu,
+ *       double& PQ, double rad)          Gas_Dinamic.cu:249-250
+ *   HLLCQ_Aleksashov(same arguments)    Gas_Dinamic.cu:355-356 (zero field)
+ * and the scalar of the live solver HLLC_2d_Korolkov_b_s (Gas_Dinamic.cu:
+ * 1604, whose PQ HLLC_2d_Korolkov computes with Q = 1 and drops), over n
+ * faces: in = 10 planes (the 8 of ast_flux_f64, then Q_L, Q_R), out = 6
+ * planes (the 5 of ast_flux_f64, then PQ).  method: AST_HLLC (the live
+ * solver's wave speeds), AST_HLLCQ or AST_HLLCQ_A (per-side wave speeds;
+ * with a zero field the two reference operators are the same HLLC, and the
+ * same as HLLC_Aleksashov, AST_HLLC_A). */
+int ast_flux_q_f64(int method, size_t n, const double* in, double n1, double n2,
+                   double rad, double gamma, double krit, double* out, void* stream);
 
 /* ---- the step ------------------------------------------------------------
  * Replaces
@@ -212,6 +240,28 @@ int ast_flux_f32_info(int method, size_t n, const float* in, float n1, float n2,
  * stopping rule is the reference's (|dp/pee| < 1e-9, pee = PVRS estimate),
  * so the guess changes the iteration count only.  info as above (FP64
  * faces never set the island flag). */
+/* The step with the neutral-H source terms of
+ *   __global__ void add_MK(double2* s, double2* u, double2* s2, double2* u2,
+ *       double* nn1, double3* nn2, double* nn3, double* T, double* T_do,
+ *       int method, int step)   kernel.cu:2293
+ * behind it (kernel.cu:2495-2498): the momentum of each cell gains
+ * T_do*(rho*kmom)*nn2.{x,y} and its energy T_do*ken*nn3, before the density
+ * and pressure floors; frozen cells are copied.  nn2 holds 3 doubles per cell
+ * (the reference's double3 plane, x and y read), nn3 one double per cell
+ * (device memory, N*M cells, row-major like s); either may be NULL (no such
+ * term); nn1 is not read by the reference's step and has no slot.  The
+ * reference's coefficients are kmom = n_H/Kn and ken = 0.0*n_H/Kn
+ * (Header.h:84-85).  src == NULL is ast_step_f64. */
+typedef struct ast_sources {
+  const double* nn2;
+  const double* nn3;
+  double kmom;
+  double ken;
+} ast_sources_t;
+int ast_step_f64_src(const ast_grid_t* g, const double* s, const double* u,
+                     double* s2, double* u2, double* T, const double* T_do,
+                     double* TT, int method, const ast_sources_t* src,
+                     ast_stats_t* stats, void* stream);
 int ast_flux_f64_guess(size_t n, const double* in, double n1, double n2,
                        double rad, double gamma, double krit, const float* guess,
                        double* out, ast_stats_t* stats, int* info, void* stream);
@@ -288,11 +338,25 @@ int ast_sim_get_stats(ast_sim_t* sim, ast_stats_t* out, int reset);
 int ast_sim_collect_stats(ast_sim_t* sim, int on);
 /* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step.
  * The network costs the solver kernel registers (occupancy) whether or not a
- * face uses it, so the simulation runs it only where it can serve: each step
- * launches the plain solver and the solver with the learned start, gated by
- * a device word, and only one of them runs -- the learned one in the steps
- * that follow a step with strong two-shock faces. */
+ * face uses it, so the simulation runs it only where it can serve: each run
+ * of 20 steps launches one solver kernel, the plain one (which flags strong
+ * two-shock faces) or the one with the learned start, chosen from the flag
+ * of the run two before (read after that run's completion, never waiting
+ * for the run in flight); the first two runs after a create or load take
+ * the plain solver (or the modes the checkpoint recorded). */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
+/* Neutral-H source planes for every later step (ast_step_f64_src), copied
+ * from host memory: nn2 N*M triples, nn3 N*M doubles, either may be NULL,
+ * both NULL removes them.  FP64 simulations only (else AST_ERR_ARG). */
+int ast_sim_set_sources(ast_sim_t* sim, const double* nn2, const double* nn3,
+                        double kmom, double ken);
+/* The learned modes (0 plain solver, 1 learned start) the next two runs
+ * will take; waits for every launched run. */
+int ast_sim_learned_modes(ast_sim_t* sim, int* modes);
+/* -1 (default): the automatic choice above; 0 / 1: every run takes the plain
+ * solver / the learned start (for strong faces that last too few steps for
+ * the automatic choice to meet them). */
+int ast_sim_set_learned_mode(ast_sim_t* sim, int mode);
 /* Strip tiling of the step.  FP32: rows > 0 rows per wave strip, rows = -S
  * S strips of (nearly) equal height per band of 60 columns; strips of 4 to
  * 64 rows inside the grid take the fast strip path.  FP64: rows > 0 rows

@@ -188,12 +188,35 @@ double ora_hll(const ora_phys* ph, const double* Ls, const double* Lu,
 
 /* HLLC_2d_Korolkov (Gas_Dinamic.cu:1814-1826) -> HLLC_2d_Korolkov_b_s
  * (Gas_Dinamic.cu:1604-1811) with Q_L = Q_R = 1, W = 0, metod = 2,
- * nul_potok = false.  The passive scalar (index 4), UZ0..UZ4 and sqrtro*
- * have no effect on the returned values under these arguments and are not
- * restated. */
+ * nul_potok = false.  UZ0..UZ4 and sqrtro* have no effect on the returned
+ * values under these arguments and are not restated; the passive scalar
+ * (index 4) is carried for ora_hllc_q. */
+static double o_hllc_q(const ora_phys* ph, const double* Ls, const double* Lu,
+                       const double* Rs, const double* Ru, double Q_L, double Q_R,
+                       double n1, double n2, double* Ps, double* Pu, double* PQ,
+                       double rad);
+
 double ora_hllc(const ora_phys* ph, const double* Ls, const double* Lu,
                 const double* Rs, const double* Ru, double n1, double n2,
                 double* Ps, double* Pu, double rad) {
+  double PQ;
+  return o_hllc_q(ph, Ls, Lu, Rs, Ru, 1.0, 1.0, n1, n2, Ps, Pu, &PQ, rad);
+}
+
+/* The live solver with its passive scalar (index 4 of FL/FR/UL/UR, Q_LL,
+ * Q_RR and PQ of Gas_Dinamic.cu:1604-1811): the scalar's flux PQ that
+ * HLLC_2d_Korolkov computes with Q_L = Q_R = 1 and drops. */
+double ora_hllc_q(const ora_phys* ph, const double* Ls, const double* Lu,
+                  const double* Rs, const double* Ru, double Q_L, double Q_R,
+                  double n1, double n2, double* Ps, double* Pu, double* PQ,
+                  double rad) {
+  return o_hllc_q(ph, Ls, Lu, Rs, Ru, Q_L, Q_R, n1, n2, Ps, Pu, PQ, rad);
+}
+
+static double o_hllc_q(const ora_phys* ph, const double* Ls, const double* Lu,
+                       const double* Rs, const double* Ru, double Q_L, double Q_R,
+                       double n1, double n2, double* Ps, double* Pu, double* PQ,
+                       double rad) {
   const double ggg = ph->ga, g1 = ph->g1;
   const double W = 0.0;
   double ro_L = Ls[0], p_L = Ls[1], v1_L = Lu[0], v2_L = Lu[1];
@@ -243,11 +266,15 @@ double ora_hllc(const ora_phys* ph, const double* Ls, const double* Lu,
   UR[2] = ro_R * v2;
   UR[3] = e2;
 
+  /* the scalar's flux and conserved value (index 4) */
+  const double FL4 = Q_L * u1, FR4 = Q_R * u2, UL4 = Q_L, UR4 = Q_R;
+
   if (SL >= W) {
     P[1] = n1 * (FL[1] - W * UL[1]) + t1 * (FL[2] - W * UL[2]);
     P[2] = n2 * (FL[1] - W * UL[1]) + t2 * (FL[2] - W * UL[2]);
     P[0] = FL[0] - W * UL[0];
     P[3] = FL[3] - W * UL[3];
+    *PQ = FL4 - W * UL4;
     goto out;
   }
   if (SR <= W) {
@@ -255,11 +282,14 @@ double ora_hllc(const ora_phys* ph, const double* Ls, const double* Lu,
     P[2] = n2 * (FR[1] - W * UR[1]) + t2 * (FR[2] - W * UR[2]);
     P[0] = FR[0] - W * UR[0];
     P[3] = FR[3] - W * UR[3];
+    *PQ = FR4 - W * UR4;
     goto out;
   }
   {
     double ro_LL = ro_L * (SL - u1) / (SL - SM);
     double ro_RR = ro_R * (SR - u2) / (SR - SM);
+    double Q_LL = Q_L * (SL - u1) / (SL - SM);
+    double Q_RR = Q_R * (SR - u2) / (SR - SM);
     double suRm = suR / (SR - SM);
     double suLm = suL / (SL - SM);
     double ptzR = p_R + ro_R * suR * (SM - u2);
@@ -274,12 +304,15 @@ double ora_hllc(const ora_phys* ph, const double* Ls, const double* Lu,
     if (SL < W && SM >= W) {
       for (int i = 0; i < 4; i++)
         PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
+      *PQ = FL4 + SL * Q_LL - SL * UL4 - W * Q_LL;
     } else if (SR > W && SM < W) {
       for (int i = 0; i < 4; i++)
         PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
+      *PQ = FR4 + SR * Q_RR - SR * UR4 - W * Q_RR;
     } else {
       /* SM is NaN: the reference leaves PO uninitialised. */
       for (int i = 0; i < 4; i++) PO[i] = NAN;
+      *PQ = NAN;
     }
     P[1] = n1 * PO[1] + t1 * PO[2];
     P[2] = n2 * PO[1] + t2 * PO[2];
@@ -309,9 +342,37 @@ out:
  * Wave speeds: SL = min(vL - cfL, vR - cfR), SR = max(vL + cfL, vR + cfR)
  * (kernel.cu:895-896), star energy with the averaged ptz (kernel.cu:981-985,
  * 1006-1007), frame speed wv = 0. */
+static double o_hllc_a_q(const ora_phys* ph, const double* Ls, const double* Lu,
+                         const double* Rs, const double* Ru, double Q_L, double Q_R,
+                         double n1, double n2, double* Ps, double* Pu, double* PQ,
+                         double rad);
+
 double ora_hllc_a(const ora_phys* ph, const double* Ls, const double* Lu,
                   const double* Rs, const double* Ru, double n1, double n2,
                   double* Ps, double* Pu, double rad) {
+  double PQ;
+  return o_hllc_a_q(ph, Ls, Lu, Rs, Ru, 0.0, 0.0, n1, n2, Ps, Pu, &PQ, rad);
+}
+
+/* HLLCQ_Aleksashov, Gas_Dinamic.cu:355-675, with B = 0 and w = 0: its
+ * frame (the else branch of Gas_Dinamic.cu:419-448 picks the z axis, so the
+ * in-plane tangent is (-n2, n1)), fast speeds, contact speed, star states and
+ * branch tests are those of HLLC_Aleksashov above (Gas_Dinamic.cu:452-625
+ * against kernel.cu:863-1031), so Ps, Pu are computed by the same lines;
+ * the passive scalar's flux is FL0 = Q_L vL0 (FR0 likewise) outside the
+ * waves and FL0 + SL (rzL Q_L/r1 - Q_L) (FR0 + SR (rzR Q_R/r2 - Q_R)) in the
+ * star regions (Gas_Dinamic.cu:531, 542, 627-665). */
+double ora_hllcq_a(const ora_phys* ph, const double* Ls, const double* Lu,
+                   const double* Rs, const double* Ru, double Q_L, double Q_R,
+                   double n1, double n2, double* Ps, double* Pu, double* PQ,
+                   double rad) {
+  return o_hllc_a_q(ph, Ls, Lu, Rs, Ru, Q_L, Q_R, n1, n2, Ps, Pu, PQ, rad);
+}
+
+static double o_hllc_a_q(const ora_phys* ph, const double* Ls, const double* Lu,
+                         const double* Rs, const double* Ru, double Q_L, double Q_R,
+                         double n1, double n2, double* Ps, double* Pu, double* PQ,
+                         double rad) {
   const double ga = ph->ga, g1 = ph->g1;
   const double r1 = Ls[0], p1 = Ls[1], u1 = Lu[0], v1 = Lu[1];
   const double r2 = Rs[0], p2 = Rs[1], u2 = Ru[0], v2 = Ru[1];
@@ -363,16 +424,22 @@ double ora_hllc_a(const ora_phys* ph, const double* Ls, const double* Lu,
   double UZL[4] = {rzL, SM * rzL, vL2 * rzL, ezL};
   double UZR[4] = {rzR, SM * rzR, vR2 * rzR, ezR};
   double P[4];
+  const double FL0 = Q_L * vL0, FR0 = Q_R * vR0;
   if (SL > 0.0) {                        /* kernel.cu:1033-1041 */
     for (int k = 0; k < 4; k++) P[k] = FL[k];
+    *PQ = FL0;
   } else if (SL <= 0.0 && SM >= 0.0) {   /* kernel.cu:1042-1050 */
     for (int k = 0; k < 4; k++) P[k] = FL[k] + SL * (UZL[k] - UL[k]);
+    *PQ = FL0 + SL * (rzL * Q_L / r1 - Q_L);
   } else if (SM <= 0.0 && SR >= 0.0) {   /* kernel.cu:1051-1059 */
     for (int k = 0; k < 4; k++) P[k] = FR[k] + SR * (UZR[k] - UR[k]);
+    *PQ = FR0 + SR * (rzR * Q_R / r2 - Q_R);
   } else if (SR < 0.0) {                 /* kernel.cu:1060-1068 */
     for (int k = 0; k < 4; k++) P[k] = FR[k];
+    *PQ = FR0;
   } else {                               /* NaN speeds: the reference prints */
     for (int k = 0; k < 4; k++) P[k] = NAN;
+    *PQ = NAN;
   }
   Ps[0] = P[0];
   Ps[1] = P[3];
@@ -380,6 +447,111 @@ double ora_hllc_a(const ora_phys* ph, const double* Ls, const double* Lu,
   Pu[0] = n1 * P[1] - n2 * P[2];
   Pu[1] = n2 * P[1] + n1 * P[2];
   return time;
+}
+
+/* HLLCQ_Korolkov_2D, Gas_Dinamic.cu:249-353: HLLC with the per-side wave
+ * speeds SL = min(uL - cL, uR - cR), SR = max(uL + cL, uR + cR), the left
+ * star pressure pp on both sides and fluxes in the F + S (U* - U) form, with
+ * a passive scalar Q (a conserved density: its star value is Q rr/rho).
+ * Tangent (-n2, n1).  When SM is NaN the reference prints and returns
+ * without writing the fluxes; here they are NaN. */
+double ora_hllcq(const ora_phys* ph, const double* Ls, const double* Lu,
+                 const double* Rs, const double* Ru, double Q_L, double Q_R,
+                 double n1, double n2, double* Ps, double* Pu, double* PQ,
+                 double rad) {
+  const double ga = ph->ga, g1 = ph->g1;
+  const double ro1 = Ls[0], p1 = Ls[1], u1 = Lu[0], v1 = Lu[1];
+  const double ro2 = Rs[0], p2 = Rs[1], u2 = Ru[0], v2 = Ru[1];
+  const double t1 = -n2, t2 = n1;
+  const double u_L = u1 * n1 + v1 * n2, v_L = u1 * t1 + v1 * t2;
+  const double u_R = u2 * n1 + v2 * n2, v_R = u2 * t1 + v2 * t2;
+  const double cL = sqrt(ga * p1 / ro1), cR = sqrt(ga * p2 / ro2);
+  const double SL = fmin(u_L - cL, u_R - cR);
+  const double SR = fmax(u_L + cL, u_R + cR);
+  const double UU = fmax(fabs(SL), fabs(SR));
+  const double time = ph->krit * rad / UU;
+  const double FL1 = ro1 * u_L * u_L + p1, FL2 = ro1 * u_L * v_L;
+  const double FR1 = ro2 * u_R * u_R + p2, FR2 = ro2 * u_R * v_R;
+  if (SL >= 0.0) {
+    *PQ = Q_L * u_L;
+    Ps[0] = ro1 * u_L;
+    Ps[1] = (ga * p1 / g1 + 0.5 * ro1 * (u1 * u1 + v1 * v1)) * u_L;
+    Pu[0] = FL1 * n1 + FL2 * t1;
+    Pu[1] = FL1 * n2 + FL2 * t2;
+    return time;
+  }
+  if (SR <= 0.0) {
+    *PQ = Q_R * u_R;
+    Ps[0] = ro2 * u_R;
+    Ps[1] = (ga * p2 / g1 + 0.5 * ro2 * (u2 * u2 + v2 * v2)) * u_R;
+    Pu[0] = FR1 * n1 + FR2 * t1;
+    Pu[1] = FR1 * n2 + FR2 * t2;
+    return time;
+  }
+  const double SM = ((SR - u_R) * ro2 * u_R - (SL - u_L) * ro1 * u_L - p2 + p1) /
+                    ((SR - u_R) * ro2 - (SL - u_L) * ro1);
+  const double pp = p1 + ro1 * (SL - u_L) * (SM - u_L);
+  if (SM <= 0.0) {
+    const double rr = ro2 * (SR - u_R) / (SR - SM);
+    const double e = p2 / g1 + 0.5 * ro2 * (u2 * u2 + v2 * v2);
+    const double ee = ((SR - u_R) * e - p2 * u_R + pp * SM) / (SR - SM);
+    *PQ = SR * (rr * Q_R / ro2 - Q_R) + Q_R * u_R;
+    Ps[0] = SR * (rr - ro2) + ro2 * u_R;
+    Ps[1] = SR * (ee - e) + (ga * p2 / g1 + 0.5 * ro2 * (u2 * u2 + v2 * v2)) * u_R;
+    const double F1 = FR1 + SR * (rr * SM - ro2 * u_R);
+    const double F2 = FR2 + SR * (rr * v_R - ro2 * v_R);
+    Pu[0] = F1 * n1 + F2 * t1;
+    Pu[1] = F1 * n2 + F2 * t2;
+  } else if (SM >= 0.0) {
+    const double rr = ro1 * (SL - u_L) / (SL - SM);
+    const double e = p1 / g1 + 0.5 * ro1 * (u1 * u1 + v1 * v1);
+    const double ee = ((SL - u_L) * e - p1 * u_L + pp * SM) / (SL - SM);
+    *PQ = SL * (rr * Q_L / ro1 - Q_L) + Q_L * u_L;
+    Ps[0] = SL * (rr - ro1) + ro1 * u_L;
+    Ps[1] = SL * (ee - e) + (ga * p1 / g1 + 0.5 * ro1 * (u1 * u1 + v1 * v1)) * u_L;
+    const double F1 = FL1 + SL * (rr * SM - ro1 * u_L);
+    const double F2 = FL2 + SL * (rr * v_L - ro1 * v_L);
+    Pu[0] = F1 * n1 + F2 * t1;
+    Pu[1] = F1 * n2 + F2 * t2;
+  } else {
+    *PQ = Ps[0] = Ps[1] = Pu[0] = Pu[1] = NAN;
+  }
+  return time;
+}
+
+/* Batched faces with a passive scalar: in = 10 planes (the 8 of
+ * ora_flux_batch, then Q_L, Q_R), out = 6 planes (the 5 of ora_flux_batch,
+ * then PQ).  method: AST_HLLC (ora_hllc_q), AST_HLLCQ (ora_hllcq),
+ * AST_HLLCQ_A (ora_hllcq_a). */
+void ora_flux_q_batch(int method, double gamma, double krit, long n,
+                      const double* in, double n1, double n2, double rad,
+                      double* out, int nthreads) {
+  ora_phys ph;
+  ora_phys_init(&ph, gamma, krit);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (long i = 0; i < n; i++) {
+    double Ls[2] = {in[0 * n + i], in[1 * n + i]};
+    double Lu[2] = {in[2 * n + i], in[3 * n + i]};
+    double Rs[2] = {in[4 * n + i], in[5 * n + i]};
+    double Ru[2] = {in[6 * n + i], in[7 * n + i]};
+    const double QL = in[8 * n + i], QR = in[9 * n + i];
+    double Ps[2], Pu[2], PQ, t;
+    if (method == AST_HLLCQ)
+      t = ora_hllcq(&ph, Ls, Lu, Rs, Ru, QL, QR, n1, n2, Ps, Pu, &PQ, rad);
+    else if (method == AST_HLLCQ_A)
+      t = ora_hllcq_a(&ph, Ls, Lu, Rs, Ru, QL, QR, n1, n2, Ps, Pu, &PQ, rad);
+    else
+      t = ora_hllc_q(&ph, Ls, Lu, Rs, Ru, QL, QR, n1, n2, Ps, Pu, &PQ, rad);
+    out[0 * n + i] = Ps[0];
+    out[1 * n + i] = Ps[1];
+    out[2 * n + i] = Pu[0];
+    out[3 * n + i] = Pu[1];
+    out[4 * n + i] = t;
+    out[5 * n + i] = PQ;
+  }
 }
 
 /* lev, kernel.cu:751-765 */
@@ -771,7 +943,7 @@ void ora_godunov_guess_batch(double gamma, double krit, long n, const double* in
  * expressed per side.  Only called with exactly one index out of range
  * (the stencil is a cross). */
 static void o_fetch(const ast_grid_t* g, const d2* s, const d2* u, int n,
-                    int m, d2* so, d2* uo) {
+                    int m, double TT, d2* so, d2* uo) {
   const int N = g->N, M = g->M;
   int side = -1, in_n = n, in_m = m;
   if (n < 0) { side = AST_XLO; in_n = -1 - n; }
@@ -784,11 +956,17 @@ static void o_fetch(const ast_grid_t* g, const d2* s, const d2* u, int n,
     return;
   }
   int bc = g->bc[side];
-  if (bc == AST_BC_DIRICHLET) {
+  if (bc == AST_BC_DIRICHLET || bc == AST_BC_DIRICHLET_PULSED) {
     so->x = g->bc_state[side][0];
     so->y = g->bc_state[side][1];
     uo->x = g->bc_state[side][2];
     uo->y = g->bc_state[side][3];
+    /* add2, kernel.cu:2022-2025: in rows below y_lim, while TT < t_lim, the
+     * inflow density is multiplied by 1 + a sin(TT w) (there a = 0.01,
+     * w = 2 pi, y_lim = 0.5, t_lim = 3) */
+    if (bc == AST_BC_DIRICHLET_PULSED && g->y_min + m * g->dy < g->bc_pulse[2] &&
+        TT < g->bc_pulse[3])
+      so->x = so->x * (1.0 + g->bc_pulse[0] * sin(TT * g->bc_pulse[1]));
     return;
   }
   if (bc == AST_BC_REFLECT) {
@@ -836,8 +1014,8 @@ typedef double (*o_flux_fn)(const ora_phys*, const double*, const double*,
  * kernel.cu:3059-3073 (planar) or EnergyConservationLaws
  * (KorolkovSolver.cu:13-32, axisymmetric).  Returns the cell's tmin. */
 static double o_cell(const ast_grid_t* g, const ora_phys* ph, const d2* s,
-                     const d2* u, d2* s2, d2* u2, double T_do, int method,
-                     int n, int m) {
+                     const d2* u, d2* s2, d2* u2, double T_do, double TT,
+                     int method, const ora_src* src, int n, int m) {
   const int N = g->N;
   const double dx = g->dx, dy = g->dy, ggg = ph->ga;
   const long index = (long)m * N + n;
@@ -862,14 +1040,14 @@ static double o_cell(const ast_grid_t* g, const ora_phys* ph, const d2* s,
   }
 
   /* gather, kernel.cu:2549-2698 */
-  o_fetch(g, s, u, n + 1, m, &s_2, &u_2);
-  o_fetch(g, s, u, n, m + 1, &s_5, &u_5);
-  o_fetch(g, s, u, n - 1, m, &s_4, &u_4);
-  o_fetch(g, s, u, n, m - 1, &s_3, &u_3);
-  o_fetch(g, s, u, n + 2, m, &s21, &u21);
-  o_fetch(g, s, u, n - 2, m, &s41, &u41);
-  o_fetch(g, s, u, n, m + 2, &s51, &u51);
-  o_fetch(g, s, u, n, m - 2, &s31, &u31);
+  o_fetch(g, s, u, n + 1, m, TT, &s_2, &u_2);
+  o_fetch(g, s, u, n, m + 1, TT, &s_5, &u_5);
+  o_fetch(g, s, u, n - 1, m, TT, &s_4, &u_4);
+  o_fetch(g, s, u, n, m - 1, TT, &s_3, &u_3);
+  o_fetch(g, s, u, n + 2, m, TT, &s21, &u21);
+  o_fetch(g, s, u, n - 2, m, TT, &s41, &u41);
+  o_fetch(g, s, u, n, m + 2, TT, &s51, &u51);
+  o_fetch(g, s, u, n, m - 2, TT, &s31, &u31);
 
   /* own-side face states, kernel.cu:2701-2797 (grad_p = true) */
   o_linear2(x - dx, s_4.x, x, s_1.x, x + dx, s_2.x, x - dx / 2.0, x + dx / 2.0, &A, &B);
@@ -953,26 +1131,35 @@ static double o_cell(const ast_grid_t* g, const ora_phys* ph, const d2* s,
   double PUy = Pu12[1] * dy + Pu13[1] * dx + Pu14[1] * dy + Pu15[1] * dx;
   double dV = dx * dy;
   d2 r, v;
+  /* neutral-H sources of add_MK, kernel.cu:2495-2498: momentum
+   * T_do*(rho*kmom)*nn2.{x,y}, energy T_do*ken*nn3 (zero without planes) */
+  double Smx = 0, Smy = 0, SE = 0;
+  if (src && src->nn2) {
+    Smx = T_do * (s_1.x * src->kmom) * src->nn2[3 * index];
+    Smy = T_do * (s_1.x * src->kmom) * src->nn2[3 * index + 1];
+  }
+  if (src && src->nn3) SE = T_do * src->ken * src->nn3[index];
 
   if (g->geometry == AST_AXISYMMETRIC) {
     /* EnergyConservationLaws, KorolkovSolver.cu:13-32 */
     r.x = s_1.x - (T_do / dV) * PSx - T_do * s_1.x * u_1.y / y;
     if (r.x <= 0) r.x = s_1.x;
-    v.x = (s_1.x * u_1.x - (T_do / dV) * PUx - T_do * s_1.x * u_1.y * u_1.x / y) / r.x;
-    v.y = (s_1.x * u_1.y - (T_do / dV) * PUy - T_do * s_1.x * u_1.y * u_1.y / y) / r.x;
+    v.x = (s_1.x * u_1.x - (T_do / dV) * PUx - T_do * s_1.x * u_1.y * u_1.x / y + Smx) / r.x;
+    v.y = (s_1.x * u_1.y - (T_do / dV) * PUy - T_do * s_1.x * u_1.y * u_1.y / y + Smy) / r.x;
     r.y = (((s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) -
             (T_do / dV) * PSy -
-            T_do * u_1.y * (ggg * s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) / y) -
+            T_do * u_1.y * (ggg * s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) / y +
+            SE) -
            0.5 * r.x * (v.x * v.x + v.y * v.y)) * (ggg - 1);
     if (r.y <= 0) r.y = 0.000001;
   } else {
     /* kernel.cu:3059-3073 with Q2 = Q2x = Q3 = 0 */
     r.x = s_1.x - (T_do / dV) * PSx;
     if (r.x <= 0.0) r.x = s_1.x;
-    v.x = (s_1.x * u_1.x - (T_do / dV) * PUx) / r.x;
-    v.y = (s_1.x * u_1.y - (T_do / dV) * PUy) / r.x;
+    v.x = (s_1.x * u_1.x - (T_do / dV) * PUx + Smx) / r.x;
+    v.y = (s_1.x * u_1.y - (T_do / dV) * PUy + Smy) / r.x;
     r.y = (((s_1.y / (ggg - 1) + s_1.x * (u_1.x * u_1.x + u_1.y * u_1.y) * 0.5) -
-            (T_do / dV) * PSy) -
+            (T_do / dV) * PSy + SE) -
            0.5 * r.x * (v.x * v.x + v.y * v.y)) * (ggg - 1);
     if (r.y <= 0.0) r.y = 0.000001;
   }
@@ -987,6 +1174,14 @@ static double o_cell(const ast_grid_t* g, const ora_phys* ph, const d2* s,
 void ora_step(const ast_grid_t* g, const double* s, const double* u,
               double* s2, double* u2, double* T, const double* T_do,
               int method, int nthreads) {
+  ora_step_src(g, s, u, s2, u2, T, T_do, 0.0, method, NULL, nthreads);
+}
+
+/* The step at simulation time TT (read by pulsed Dirichlet sides) with the
+ * neutral-H source planes of add_MK (src may be NULL). */
+void ora_step_src(const ast_grid_t* g, const double* s, const double* u,
+                  double* s2, double* u2, double* T, const double* T_do,
+                  double TT, int method, const ora_src* src, int nthreads) {
   ora_phys ph;
   ora_phys_init(&ph, g->gamma, g->krit);
   double tmin_all = *T;
@@ -998,7 +1193,7 @@ void ora_step(const ast_grid_t* g, const double* s, const double* u,
   for (int m = 0; m < g->M; m++) {
     for (int n = 0; n < g->N; n++) {
       double t = o_cell(g, &ph, (const d2*)s, (const d2*)u, (d2*)s2, (d2*)u2,
-                        tdo, method, n, m);
+                        tdo, TT, method, src, n, m);
       if (tmin_all > t) tmin_all = t;
     }
   }
@@ -1021,7 +1216,7 @@ double ora_step_rows(const ast_grid_t* g, const double* s, const double* u,
   for (int m = m0; m < m1; m++) {
     for (int n = 0; n < g->N; n++) {
       double t = o_cell(g, &ph, (const d2*)s, (const d2*)u, (d2*)s2, (d2*)u2,
-                        tdo, method, n, m);
+                        tdo, 0.0, method, NULL, n, m);
       if (tmin_all > t) tmin_all = t;
     }
   }

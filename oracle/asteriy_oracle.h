@@ -34,6 +34,22 @@ double ora_hllc(const ora_phys*, const double* Ls, const double* Lu,
 double ora_hllc_a(const ora_phys*, const double* Ls, const double* Lu,
                 const double* Rs, const double* Ru, double n1, double n2,
                 double* Ps, double* Pu, double rad);
+/* passive-scalar fluxes: (Ps, Pu, dt) as the operators above plus PQ */
+double ora_hllc_q(const ora_phys*, const double* Ls, const double* Lu,
+                  const double* Rs, const double* Ru, double QL, double QR,
+                  double n1, double n2, double* Ps, double* Pu, double* PQ,
+                  double rad);
+double ora_hllcq(const ora_phys*, const double* Ls, const double* Lu,
+                 const double* Rs, const double* Ru, double QL, double QR,
+                 double n1, double n2, double* Ps, double* Pu, double* PQ,
+                 double rad);
+double ora_hllcq_a(const ora_phys*, const double* Ls, const double* Lu,
+                   const double* Rs, const double* Ru, double QL, double QR,
+                   double n1, double n2, double* Ps, double* Pu, double* PQ,
+                   double rad);
+void ora_flux_q_batch(int method, double gamma, double krit, long n,
+                      const double* in, double n1, double n2, double rad,
+                      double* out, int nthreads);
 double ora_godunov(const ora_phys*, const double* Ls, const double* Lu,
                    const double* Rs, const double* Ru, double n1, double n2,
                    double* Ps, double* Pu, double rad);
@@ -54,6 +70,16 @@ void ora_godunov_guess_batch(double gamma, double krit, long n, const double* in
 void ora_step(const ast_grid_t* g, const double* s, const double* u,
               double* s2, double* u2, double* T, const double* T_do,
               int method, int nthreads);
+/* neutral-H source planes of add_MK (kernel.cu:2495-2498): nn2 holds 3
+ * doubles per cell (x, y used), nn3 one; either may be NULL */
+typedef struct {
+  const double* nn2;
+  const double* nn3;
+  double kmom, ken;
+} ora_src;
+void ora_step_src(const ast_grid_t* g, const double* s, const double* u,
+                  double* s2, double* u2, double* T, const double* T_do,
+                  double TT, int method, const ora_src* src, int nthreads);
 double ora_step_rows(const ast_grid_t* g, const double* s, const double* u,
                      double* s2, double* u2, const double* T_do, int method,
                      int m0, int m1, int nthreads);

@@ -94,8 +94,18 @@ def lib():
         L.ora_godunov_guess_batch.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_long,
                                               _dp, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_double, _dp, _dp, _ip, _dp]
+        for f in (L.ora_hllc_q, L.ora_hllcq, L.ora_hllcq_a):
+            f.restype = ctypes.c_double
+            f.argtypes = [ctypes.POINTER(CPhys), _dp, _dp, _dp, _dp, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
+                          ctypes.c_double]
+        L.ora_flux_q_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_long, _dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, _dp, ctypes.c_int]
         L.ora_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
                                ctypes.c_int, ctypes.c_int]
+        L.ora_step_src.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
         L.ora_step_rows.restype = ctypes.c_double
         L.ora_step_rows.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -149,6 +159,33 @@ def flux(method: int, Ls, Lu, Rs, Ru, n1, n2, rad, gamma=5.0 / 3.0, krit=0.05):
     return Ps, Pu, t
 
 
+# passive-scalar operators: AST_HLLC (the live solver's scalar), AST_HLLCQ,
+# AST_HLLCQ_A
+_FLUX_Q = {1: "ora_hllc_q", 4: "ora_hllcq", 5: "ora_hllcq_a"}
+
+
+def flux_q(method: int, Ls, Lu, Rs, Ru, QL, QR, n1, n2, rad, gamma=5.0 / 3.0, krit=0.05):
+    """One face with a passive scalar: returns (Ps, Pu, PQ, dt)."""
+    ph = phys(gamma, krit)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (Ls, Lu, Rs, Ru)]
+    Ps, Pu, PQ = np.zeros(2), np.zeros(2), np.zeros(1)
+    t = getattr(lib(), _FLUX_Q[method])(ctypes.byref(ph), *[_p(v) for v in a], QL, QR, n1, n2,
+                                         _p(Ps), _p(Pu), _p(PQ), rad)
+    return Ps, Pu, float(PQ[0]), t
+
+
+def flux_q_batch(method: int, faces: np.ndarray, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0,
+                 krit=0.05, nthreads: int = 0):
+    """faces: (10, n) SoA (the 8 planes of flux_batch, Q_L, Q_R) -> out (6, n)
+    (the 5 planes of flux_batch, PQ)."""
+    faces = np.ascontiguousarray(faces, dtype=np.float64)
+    assert faces.shape[0] == 10
+    n = faces.shape[1]
+    out = np.empty((6, n))
+    lib().ora_flux_q_batch(method, gamma, krit, n, _p(faces), n1, n2, rad, _p(out), nthreads)
+    return out
+
+
 def godunov_info(Ls, Lu, Rs, Ru, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0, krit=0.05,
                  guess=None):
     ph = phys(gamma, krit)
@@ -190,18 +227,48 @@ def godunov_guess_batch(faces: np.ndarray, guess: np.ndarray | None, n1=1.0, n2=
     return out, it, ps
 
 
+class CSrc(ctypes.Structure):
+    _fields_ = [("nn2", ctypes.c_void_p), ("nn3", ctypes.c_void_p),
+                ("kmom", ctypes.c_double), ("ken", ctypes.c_double)]
+
+
+class Sources:
+    """Neutral-H source planes of add_MK (kernel.cu:2495-2498): nn2 (M,N,3)
+    and nn3 (M,N) float64 (either may be None), momentum coefficient kmom and
+    energy coefficient ken (the reference: n_H/Kn and 0.0*n_H/Kn,
+    Header.h:84-85)."""
+
+    def __init__(self, nn2=None, nn3=None, kmom=3.0 / 0.419231, ken=0.0):
+        self.nn2 = None if nn2 is None else np.ascontiguousarray(nn2, dtype=np.float64)
+        self.nn3 = None if nn3 is None else np.ascontiguousarray(nn3, dtype=np.float64)
+        self.kmom, self.ken = float(kmom), float(ken)
+
+    def to_c(self):
+        return CSrc(None if self.nn2 is None else self.nn2.ctypes.data,
+                    None if self.nn3 is None else self.nn3.ctypes.data, self.kmom, self.ken)
+
+
 def step(grid, s: np.ndarray, u: np.ndarray, T: float, T_do: float, method: int,
-         nthreads: int = 0):
-    """One Kernel_TVD step on the CPU.  s,u: (M,N,2) float64.  Returns
-    (s2, u2, T_new)."""
+         nthreads: int = 0, src: "Sources | None" = None, TT: float = 0.0):
+    """One Kernel_TVD step on the CPU.  s,u: (M,N,2) float64; src: neutral-H
+    source planes (add_MK's terms) or None; TT: the simulation time (read by
+    pulsed Dirichlet sides).  Returns (s2, u2, T_new)."""
     cg = grid.to_c()
     s = np.ascontiguousarray(s, dtype=np.float64)
     u = np.ascontiguousarray(u, dtype=np.float64)
     s2, u2 = np.empty_like(s), np.empty_like(u)
     Tv = np.array([T], dtype=np.float64)
     Td = np.array([T_do], dtype=np.float64)
-    lib().ora_step(ctypes.byref(cg), _p(s), _p(u), _p(s2), _p(u2), _p(Tv), _p(Td), method,
-                   nthreads)
+    cs = None
+    if src is not None:
+        if src.nn2 is not None:
+            assert src.nn2.shape == (grid.M, grid.N, 3)
+        if src.nn3 is not None:
+            assert src.nn3.shape == (grid.M, grid.N)
+        cs = src.to_c()
+    lib().ora_step_src(ctypes.byref(cg), _p(s), _p(u), _p(s2), _p(u2), _p(Tv), _p(Td),
+                       float(TT), method, ctypes.byref(cs) if cs is not None else None,
+                       nthreads)
     return s2, u2, float(Tv[0])
 
 
@@ -225,8 +292,8 @@ class Sim:
     """The reference's host protocol on the CPU (kernel.cu:3868-3906): step,
     then funk_time; initial T = 1e7, T_do = 1e-9, TT = 0 (kernel.cu:3321-3324)."""
 
-    def __init__(self, grid, s, u, method, nthreads=0):
-        self.grid, self.method, self.nthreads = grid, method, nthreads
+    def __init__(self, grid, s, u, method, nthreads=0, src=None):
+        self.grid, self.method, self.nthreads, self.src = grid, method, nthreads, src
         self.s = np.ascontiguousarray(s, dtype=np.float64)
         self.u = np.ascontiguousarray(u, dtype=np.float64)
         self.T, self.T_do, self.TT, self.i = 1e7, 1e-9, 0.0, 0
@@ -234,7 +301,7 @@ class Sim:
     def advance(self, steps=1):
         for _ in range(steps):
             self.s, self.u, self.T = step(self.grid, self.s, self.u, self.T, self.T_do,
-                                          self.method, self.nthreads)
+                                          self.method, self.nthreads, self.src, self.TT)
             self.T, self.T_do, self.TT, self.i = funk_time(self.T, self.T_do, self.TT, self.i)
         return self
 

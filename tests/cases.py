@@ -72,6 +72,79 @@ def uniform_axisymmetric(T_do=0.01, rho=0.8, p=1.3, ux=0.45, uy=-0.7):
     return g, s, u, s2, u2
 
 
+# the momentum coefficient of the reference's neutral-H source, n_H/Kn
+# (Header.h:84-85, kernel.cu:2495-2496)
+K_MOM_REF = 3.0 / 0.419231
+
+
+def neutral_sources(M, N, seed=7, scale=0.3, scale3=None):
+    """Random neutral-H moment planes: nn2 (M,N,3) with a z component of a
+    very different size (the step must not read it) and nn3 (M,N); entries
+    uniform in +-scale (+-scale3 for nn3)."""
+    rng = np.random.default_rng(seed)
+    nn2 = rng.uniform(-scale, scale, (M, N, 3))
+    nn2[..., 2] = rng.uniform(1e3, 2e3, (M, N))
+    s3 = scale if scale3 is None else scale3
+    nn3 = rng.uniform(-s3, s3, (M, N))
+    return nn2, nn3
+
+
+def frozen_mask(g):
+    """Cells inside the grid's frozen ellipse (copied by the step,
+    kernel.cu:2317-2324)."""
+    if not g.frozen_ax > 0:
+        return np.zeros((g.M, g.N), bool)
+    x = g.x(np.arange(g.N))[None, :]
+    y = g.y(np.arange(g.M))[:, None]
+    return ((x - g.frozen_x0) ** 2 / g.frozen_ax ** 2 + y ** 2 / g.frozen_ay ** 2) < 1.0
+
+
+def sources_case(N=40, M=24, seed=3):
+    """The wind problem with a frozen sphere that covers cells of a small
+    grid, and random source planes: (problem, nn2, nn3, frozen mask).  The
+    planes are sized so that with the reference's kmom = n_H/Kn and ken =
+    0.02 a step of this coarse grid (dt up to ~4) changes a velocity by a few
+    per cent of the inflow speed and a pressure by a tenth of the inflow
+    pressure at most -- strong enough to matter, not so strong that the
+    reference's own update (E' - rho'|u'|^2/2) cancels to round-off."""
+    from asteriy_2d_amd import problems
+    p = problems.wind(N, M, method=G.HLLC, wp=problems.WindParams(r_frozen=60.0))
+    nn2, nn3 = neutral_sources(p.grid.M, p.grid.N, seed, scale=2e-3, scale3=1e-4)
+    return p, nn2, nn3, frozen_mask(p.grid)
+
+
+def uniform_with_sources(geometry, T_do=0.01, rho=0.8, p=1.3, ux=0.45, uy=-0.7,
+                         kmom=K_MOM_REF, ken=0.7, seed=7):
+    """A uniform state on a grid with zero-gradient ghosts on every side (all
+    fluxes cancel) under the neutral-H sources of add_MK (kernel.cu:2495-2498):
+    momentum gains T_do rho kmom nn2.{x,y}, energy T_do ken nn3, and in
+    axisymmetric geometry the r-source acts as in uniform_axisymmetric.  With
+    f = T_do v / y (0 when planar): rho' = rho (1 - f),
+    u' = u + T_do kmom nn2.x / (1 - f), v' = v + T_do kmom nn2.y / (1 - f),
+    p' = (E (1 - f) - f p + T_do ken nn3 - rho'|u'|^2/2)(g - 1).  Returns
+    (grid, s, u, nn2, nn3, expected s2, expected u2)."""
+    g = G.Grid.from_extent(12, 9, -1.0, 1.0, 0.9, gamma=5.0 / 3.0, krit=0.3,
+                           geometry=geometry, bc=(G.BC_OUTFLOW,) * 4)
+    nn2, nn3 = neutral_sources(g.M, g.N, seed)
+    s = np.empty((g.M, g.N, 2))
+    s[..., 0], s[..., 1] = rho, p
+    u = np.empty_like(s)
+    u[..., 0], u[..., 1] = ux, uy
+    if geometry == G.AXISYMMETRIC:
+        y = g.y(np.arange(g.M))[:, None] * np.ones((1, g.N))
+        f = T_do * uy / y
+    else:
+        f = np.zeros((g.M, g.N))
+    gm = g.gamma
+    r2 = rho * (1 - f)
+    vx2 = ux + T_do * kmom * nn2[..., 0] / (1 - f)
+    vy2 = uy + T_do * kmom * nn2[..., 1] / (1 - f)
+    E = p / (gm - 1) + 0.5 * rho * (ux * ux + uy * uy)
+    E2 = E * (1 - f) - f * p + T_do * ken * nn3
+    p2 = (E2 - 0.5 * r2 * (vx2 * vx2 + vy2 * vy2)) * (gm - 1)
+    return g, s, u, nn2, nn3, np.stack([r2, p2], -1), np.stack([vx2, vy2], -1)
+
+
 # ---- comparison of an FP64 result with the oracle ---------------------------
 #
 # The FP64 GPU path and the oracle compute the same step with different (but
@@ -119,6 +192,14 @@ def flux_scales(F, gamma):
     return np.stack([r * v, (E + p) * v, mom, mom])
 
 
+def q_scale(Fq, gamma):
+    """Natural scale of the passive scalar's flux on faces Fq (10, n):
+    max(|Q_L|, |Q_R|) times the largest speed |u| + c of the two sides."""
+    c = np.sqrt(gamma * np.maximum(Fq[1] / Fq[0], Fq[5] / Fq[4]))
+    v = np.abs(Fq[[2, 3, 6, 7]]).max(0) + c
+    return np.maximum(np.abs(Fq[8]), np.abs(Fq[9])) * v + 1e-300
+
+
 def flux_mismatch(out, ref, F, gamma, rel=PARITY_REL):
     """(count, worst) of flux values (4 planes) differing from the oracle by
     more than rel of their natural scale, plus dt (plane 4) relative."""
@@ -129,3 +210,38 @@ def flux_mismatch(out, ref, F, gamma, rel=PARITY_REL):
     dt = np.abs(out[4] - ref[4]) / np.abs(ref[4])
     dt = np.where((out[4] == ref[4]), 0.0, np.where(np.isnan(dt), np.inf, dt))
     return int((d > rel).sum() + (dt > rel).sum()), float(max(d.max(), dt.max()))
+
+
+def pulsed_case(N=48, M=16, pulse=(0.2, 2.0 * math.pi * 100.0, 0.45, 0.004)):
+    """The reference's live configuration shrunk to N x M (Dirichlet inflow
+    {1, 7, 1, 0} at x-lo, outflow at x-hi, mirrors on y, HLLC faces inside
+    x in (10, 40)) with the x-lo side pulsed: density factor 1 + a sin(w TT)
+    in the rows with y < y_lim (about half of them) while TT < t_lim.  The
+    amplitude and frequency are raised over add2's (0.01, 2 pi) and t_lim
+    lowered from 3 so that the pulse is visible and ends within a few steps
+    (dt is about 7e-4 here).  State: the inflow state with a
+    density bump downstream.  Returns (grid, s, u)."""
+    g = G.reference_live_grid(N, M).replace(
+        bc=(G.BC_DIRICHLET_PULSED, G.BC_OUTFLOW, G.BC_REFLECT, G.BC_REFLECT), bc_pulse=pulse)
+    s = np.empty((M, N, 2))
+    u = np.zeros((M, N, 2))
+    x = g.x(np.arange(N))[None, :]
+    s[..., 0] = 1.0 + 0.5 * np.exp(-((x - 20.0) / 8.0) ** 2)
+    s[..., 1] = 7.0
+    u[..., 0] = 1.0
+    return g, s, u
+
+
+def pulsed_equivalent(g, TT):
+    """The two plain-Dirichlet grids a pulsed step at time TT must match row
+    by row: (grid for the rows with y < y_lim, grid for the others) -- the
+    first with the inflow density times 1 + a sin(TT w) while TT < t_lim
+    (add2, kernel.cu:2022-2025), the second unchanged."""
+    a, w, ylim, tlim = g.bc_pulse
+    f = (1.0 + a * math.sin(TT * w)) if TT < tlim else 1.0
+    st = [list(r) for r in g.bc_state]
+    lo = [list(r) for r in g.bc_state]
+    lo[G.XLO][0] = st[G.XLO][0] * f
+    bc = (G.BC_DIRICHLET,) + tuple(g.bc[1:])
+    rows_lo = g.y(np.arange(g.M)) < ylim
+    return g.replace(bc=bc, bc_state=lo), g.replace(bc=bc, bc_state=st), rows_lo

@@ -2,6 +2,7 @@
 function include/asteriy.h declares, validates grids without a device, and
 its struct layouts match the Python mirrors.  No compute calls."""
 import ctypes
+import math
 import os
 import re
 
@@ -58,7 +59,10 @@ def test_no_device_needed_for_validation(lib):
     bad = [g.replace(N=1), g.replace(gamma=1.0), g.replace(krit=0.0),
            g.replace(geometry=7), g.replace(bc=(9, 0, 0, 0)),
            g.replace(bc=(G.BC_DIRICHLET, 0, 0, 0), bc_state=[[0.0] * 4] * 4),
-           g.replace(geometry=G.AXISYMMETRIC, y_min=0.0)]
+           g.replace(geometry=G.AXISYMMETRIC, y_min=0.0),
+           g.replace(bc=(G.BC_DIRICHLET_PULSED, 0, 0, 0), bc_pulse=(1.0, 6.0, 0.5, 3.0)),
+           g.replace(bc=(G.BC_DIRICHLET_PULSED, 0, 0, 0), bc_pulse=(0.01, math.inf, 0.5, 3.0)),
+           g.replace(bc=(5, 0, 0, 0))]
     for b in bad:
         rc = L.ast_grid_check(native.grid_ref(b))
         assert rc != 0
@@ -67,9 +71,10 @@ def test_no_device_needed_for_validation(lib):
 
 def test_grid_struct_layout():
     from asteriy_2d_amd.grid import CGrid, CStats
-    # int32 N, M, geometry, bc[4] -> 28 bytes, padded to 32; 4+2+16+4+2+3 doubles
-    assert ctypes.sizeof(CGrid) == 32 + 8 * (6 + 16 + 4 + 2 + 3)
+    # int32 N, M, geometry, bc[4] -> 28 bytes, padded to 32; 4+2+16+4+2+3+4 doubles
+    assert ctypes.sizeof(CGrid) == 32 + 8 * (6 + 16 + 4 + 2 + 3 + 4)
     assert CGrid.x_min.offset == 32
+    assert CGrid.bc_pulse.offset == 32 + 8 * (6 + 16 + 4 + 2 + 3)
     assert ctypes.sizeof(CStats) == 10 * 8
 
 

@@ -173,11 +173,31 @@ def test_gated_learned_step_without_strong_faces_is_the_plain_step(dev):
     assert cb["mlp_faces"] == 0
 
 
-def test_gated_learned_step_serves_strong_faces(dev):
-    """Colliding streams: after the first step flags strong two-shock faces
-    the learned kernel runs (its faces start newton from the network) and the
-    result agrees with the plain solver to the FP32 newton tolerance; graph
-    replay and eager launches agree bit for bit."""
+def test_learned_mode_follows_strong_faces(dev):
+    """Colliding streams: the first run of 20 steps meets strong two-shock
+    faces, so the automatic choice (the flag of the run two before) gives the
+    runs after it (0 plain, 1 learned); a run without strong faces switches
+    the run two later back to the plain solver."""
+    from asteriy_2d_amd.sim import Simulation
+    g, s, u = _colliding()
+    model = learned.LearnedGuess.load(device=dev)
+    with Simulation(g, G.EXACT, "f32", s, u) as sim:
+        sim.set_mlp(model)
+        assert sim.learned_modes() == (0, 0)
+        sim.run(20)
+        assert sim.learned_modes() == (0, 1)
+        sim.run(20)
+        m = sim.learned_modes()
+        assert m[0] == 1
+        sim.run(60)
+        assert sim.learned_modes() == (0, 0)  # the collision has passed
+
+
+def test_learned_start_serves_strong_faces(dev):
+    """Colliding streams with the learned start on for every run: its faces
+    start newton from the network, the result agrees with the plain solver to
+    the FP32 newton tolerance, and graph replay and eager launches (the same
+    runs) agree bit for bit."""
     from asteriy_2d_amd.sim import Simulation
     g, s, u = _colliding()
     model = learned.LearnedGuess.load(device=dev)
@@ -187,6 +207,7 @@ def test_gated_learned_step_serves_strong_faces(dev):
         with Simulation(g, G.EXACT, "f32", s, u) as sim:
             if mlp is not None:
                 sim.set_mlp(mlp)
+                sim.set_learned_mode(1)
             sim.collect_stats(True)
             sim.run(25, graph=graph)  # graph: one replay of 20 steps + 5 direct
             res[tag] = (sim.state(), sim.stats())

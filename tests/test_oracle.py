@@ -21,7 +21,8 @@ import pytest
 from asteriy_2d_amd import grid as G
 from asteriy_2d_amd import facegen, problems
 
-from cases import FLOOR_REGIMES, floor_case, uniform_axisymmetric
+from cases import (FLOOR_REGIMES, K_MOM_REF, floor_case, flux_scales, q_scale, sources_case,
+                   uniform_axisymmetric, uniform_with_sources)
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -506,3 +507,162 @@ def test_nan_input_face_returns_nan_without_iterating(ora):
         Ps, Pu, _, info = ora.godunov_info(L[0], L[1], R[0], R[1])
         assert info.pattern == -9 and info.iters == 0
         assert np.isnan(Ps).all() and np.isnan(Pu).all()
+
+
+# ---- neutral-H sources of add_MK (kernel.cu:2495-2498) -------------------------
+
+@pytest.mark.parametrize("geometry", [G.PLANAR, G.AXISYMMETRIC])
+@pytest.mark.parametrize("ken", [0.0, 0.7])
+def test_neutral_sources_closed_form(ora, geometry, ken):
+    """A uniform state under random source planes: the closed form of
+    uniform_with_sources (a source without the density factor, on the wrong
+    component, of the wrong sign, read from nn2.z or added to the energy with
+    kmom fails it).  ken = 0 is the reference's own energy coefficient."""
+    T_do = 0.01
+    g, s, u, nn2, nn3, s2w, u2w = uniform_with_sources(geometry, T_do=T_do, ken=ken)
+    src = ora.Sources(nn2, nn3, K_MOM_REF, ken)
+    s2, u2, _ = ora.step(g, s, u, 1e7, T_do, G.HLLC, src=src)
+    np.testing.assert_allclose(s2[..., 0], s2w[..., 0], rtol=1e-13)
+    np.testing.assert_allclose(u2, u2w, rtol=1e-13, atol=1e-15)
+    E = s[..., 1] / (g.gamma - 1)
+    assert np.max(np.abs(s2[..., 1] - s2w[..., 1]) / E) < 1e-13
+
+
+def test_neutral_sources_superpose_on_a_live_state(ora):
+    """On a non-uniform state (the wind problem, every face a real flux) the
+    sources change only what they add: rho' is the same, rho' u' differs by
+    T_do rho kmom nn2 and E' by T_do ken nn3, frozen cells are copied; zero
+    planes (the reference's initial moments, kernel.cu:3337-3339) give the
+    source-free step bit for bit."""
+    p, nn2, nn3, fr = sources_case()
+    g, T_do, ken = p.grid, 2e-3, 0.02  # small against the inflow pressure: no floor
+    assert fr.any()
+    a_s, a_u, _ = ora.step(g, p.s, p.u, 1e7, T_do, G.HLLC)
+    b_s, b_u, _ = ora.step(g, p.s, p.u, 1e7, T_do, G.HLLC,
+                           src=ora.Sources(nn2, nn3, K_MOM_REF, ken))
+    z_s, z_u, _ = ora.step(g, p.s, p.u, 1e7, T_do, G.HLLC,
+                           src=ora.Sources(np.zeros_like(nn2), np.zeros_like(nn3), K_MOM_REF, 0.0))
+    assert np.array_equal(z_s, a_s) and np.array_equal(z_u, a_u)
+    assert np.array_equal(a_s[..., 0], b_s[..., 0])
+    assert (b_s[..., 1] > 1e-6).all()
+    assert np.array_equal(b_s[fr], p.s[fr]) and np.array_equal(b_u[fr], p.u[fr])
+    r2 = a_s[..., 0]
+    dm = r2[..., None] * (b_u - a_u)
+    want = T_do * p.s[..., 0, None] * K_MOM_REF * nn2[..., :2]
+    scale = np.abs(p.s[..., 0, None] * p.u).max()
+    assert np.max(np.abs(dm - want)[~fr]) < 1e-12 * scale
+    gm = g.gamma
+    Ea = a_s[..., 1] / (gm - 1) + 0.5 * r2 * (a_u ** 2).sum(-1)
+    Eb = b_s[..., 1] / (gm - 1) + 0.5 * r2 * (b_u ** 2).sum(-1)
+    assert np.max(np.abs((Eb - Ea) - T_do * ken * nn3)[~fr]) < 1e-12 * np.abs(Ea).max()
+
+
+# ---- passive-scalar fluxes (Gas_Dinamic.cu:249-675, 1604-1811) -----------------
+
+Q_METHODS = [G.HLLC, G.HLLCQ, G.HLLCQ_A]
+
+
+def _q_faces(n, seed, phi=None):
+    """(10, n) faces: random physical pairs and scalars Q = phi rho (a common
+    phi when given, else independent per side)."""
+    F = facegen.faces(n, seed=seed, vacuum_frac=0.0)
+    rng = np.random.default_rng(seed + 100)
+    if phi is None:
+        q = rng.uniform(-2.0, 2.0, (2, n))
+        QL, QR = q[0] * F[0], q[1] * F[4]
+    else:
+        QL, QR = phi * F[0], phi * F[4]
+    return np.concatenate([F, QL[None], QR[None]])
+
+
+@pytest.mark.parametrize("method", Q_METHODS)
+def test_scalar_moves_with_the_mass(ora, method):
+    """Q = phi rho on both sides with one phi: the exact solution carries phi
+    with every fluid element, so PQ = phi * mass flux on every face and in
+    every branch (a wrong star value of Q, a wrong side or the wrong speed
+    fails); PQ is linear in (Q_L, Q_R) and zero for zero scalars."""
+    for phi in (1.0, -0.37, 2.5):
+        Fq = _q_faces(2000, seed=21, phi=phi)
+        for n in ((1.0, 0.0), (0.0, -1.0), (0.6, 0.8)):
+            out = ora.flux_q_batch(method, Fq, *n, 0.5)
+            np.testing.assert_allclose(out[5], phi * out[0], rtol=1e-12,
+                                       atol=1e-12 * np.abs(phi * out[0]).max())
+    Fq = _q_faces(2000, seed=22)
+    a = ora.flux_q_batch(method, Fq)
+    Fq2 = Fq.copy()
+    Fq2[8:] *= -3.0
+    b = ora.flux_q_batch(method, Fq2)
+    assert (np.abs(b[5] + 3.0 * a[5]) / q_scale(Fq2, 5 / 3)).max() < 1e-13
+    Fq2[8:] = 0.0
+    assert np.all(ora.flux_q_batch(method, Fq2)[5] == 0.0)
+
+
+@pytest.mark.parametrize("method", Q_METHODS)
+def test_scalar_upwinding_and_contacts(ora, method):
+    """Supersonic faces carry the upwind side's Q u exactly; a contact
+    (equal u and p) carries the upwind side's Q u -- zero when it is at
+    rest, whatever the two scalars."""
+    Ps, Pu, PQ, _ = ora.flux_q(method, [1.0, 0.1], [5.0, 0.3], [0.3, 2.0], [9.0, -0.2],
+                               0.7, -1.1, 1.0, 0.0, 1.0)
+    assert PQ == 0.7 * 5.0
+    Ps, Pu, PQ, _ = ora.flux_q(method, [1.0, 0.1], [-9.0, 0.3], [0.3, 2.0], [-12.0, -0.2],
+                               0.7, -1.1, 1.0, 0.0, 1.0)
+    assert PQ == -1.1 * -12.0
+    for u, want in ((0.0, 0.0), (0.4, 0.9 * 0.4), (-0.4, -1.3 * -0.4)):
+        Ps, Pu, PQ, _ = ora.flux_q(method, [1.0, 1.0], [u, 0.2], [0.1, 1.0], [u, -0.3],
+                                   0.9, -1.3, 1.0, 0.0, 1.0)
+        assert abs(PQ - want) <= 1e-14, (u, PQ, want)
+
+
+def test_hllcq_variants_are_one_flux(ora):
+    """With a zero magnetic field HLLCQ_Korolkov_2D, HLLCQ_Aleksashov and
+    HLLC_Aleksashov are the same HLLC (per-side wave speeds, the same contact
+    speed and star states, differently rounded): (Ps, Pu, dt) and PQ agree to
+    round-off on random faces, and HLLCQ satisfies the flux invariants
+    (consistency with the physical flux, mirror antisymmetry)."""
+    Fq = _q_faces(3000, seed=23)
+    for n in ((1.0, 0.0), (0.0, 1.0), (-0.6, 0.8)):
+        a = ora.flux_q_batch(G.HLLCQ, Fq, *n, 0.5)
+        b = ora.flux_q_batch(G.HLLCQ_A, Fq, *n, 0.5)
+        c = ora.flux_batch(G.HLLC_A, Fq[:8], *n, 0.5)
+        sc, qs = flux_scales(Fq[:8], 5 / 3), q_scale(Fq, 5 / 3)
+        assert (np.abs(a[:4] - b[:4]) / sc).max() < 1e-12
+        assert (np.abs(a[:4] - c[:4]) / sc).max() < 1e-12
+        assert (np.abs(a[5] - b[5]) / qs).max() < 1e-12
+        assert np.allclose(a[4], c[4], rtol=1e-14) and np.allclose(b[4], c[4], rtol=1e-14)
+        Fs = np.ascontiguousarray(Fq[[4, 5, 6, 7, 0, 1, 2, 3, 9, 8]])
+        m = ora.flux_q_batch(G.HLLCQ, Fs, -n[0], -n[1], 0.5)
+        assert (np.abs(a[:4] + m[:4]) / sc).max() < 1e-12
+        assert (np.abs(a[5] + m[5]) / qs).max() < 1e-12
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        r, p = math.exp(rng.uniform(-3, 3)), math.exp(rng.uniform(-3, 3))
+        u = rng.uniform(-3, 3, 2)
+        Ps, Pu, PQ, _ = ora.flux_q(G.HLLCQ, [r, p], u, [r, p], u, 0.3 * r, 0.3 * r, 0.0, 1.0, 1.0)
+        rs, ru = phys_flux(r, p, u[0], u[1], 0.0, 1.0, 5 / 3)
+        ref = np.concatenate([rs, ru])
+        assert np.abs(np.concatenate([Ps, Pu]) - ref).max() <= 1e-13 * (np.abs(ref).max() + 1)
+        assert abs(PQ - 0.3 * rs[0]) <= 1e-13 * (abs(rs[0]) + 1)
+
+
+# ---- pulsed inflow (add2, kernel.cu:2022-2025) -------------------------------
+
+@pytest.mark.parametrize("TT", [0.0, 0.0013, 0.0031, 0.0039, 0.004, 0.02])
+def test_pulsed_inflow_row_by_row(ora, TT):
+    """A pulsed step at time TT equals, row by row, a plain Dirichlet step
+    whose inflow density is scaled by 1 + a sin(w TT) (rows below y_lim,
+    while TT < t_lim) or unscaled (the other rows, and every row from t_lim
+    on), bit for bit: each cell reads the ghosts of its own row only.  The
+    modulation changes the rows below y_lim for a non-zero sine."""
+    from cases import pulsed_case, pulsed_equivalent
+    g, s, u = pulsed_case()
+    g_lo, g_hi, rows_lo = pulsed_equivalent(g, TT)
+    assert rows_lo.any() and not rows_lo.all()
+    ps, pu, pT = ora.step(g, s, u, 1e7, 0.01, G.HLLC, TT=TT)
+    ls, lu, lT = ora.step(g_lo, s, u, 1e7, 0.01, G.HLLC)
+    hs, hu, hT = ora.step(g_hi, s, u, 1e7, 0.01, G.HLLC)
+    assert np.array_equal(ps[rows_lo], ls[rows_lo]) and np.array_equal(pu[rows_lo], lu[rows_lo])
+    assert np.array_equal(ps[~rows_lo], hs[~rows_lo]) and np.array_equal(pu[~rows_lo], hu[~rows_lo])
+    assert pT == min(lT, hT)
+    moved = not np.array_equal(ls, hs)
+    assert moved == (TT < g.bc_pulse[3] and math.sin(TT * g.bc_pulse[1]) != 0.0)
