@@ -59,10 +59,10 @@ def _timed(runner, steps: int, stream):
     return e0.elapsed_time(e1)
 
 
-def _kernel_ms(grid, precision, method, stream, krows=None, reps=50):
+def _kernel_ms(grid, precision, method, stream, krows=None, reps=50, sources=None):
     """Average duration of the step kernel alone: `reps` back-to-back launches
     on `stream` between two HIP events (the queue stays full, so the launch
-    gaps are hidden)."""
+    gaps are hidden).  sources: neutral-H planes for the FP64 step."""
     import torch
 
     from asteriy_2d_amd import native, problems, solver
@@ -81,12 +81,12 @@ def _kernel_ms(grid, precision, method, stream, krows=None, reps=50):
         L.ast_set_tile_rows(int(krows))
     torch.cuda.synchronize()
     for _ in range(3):
-        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream)
+        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream, sources=sources)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream)
+        solver.Kernel_TVD(grid, S, U, S2, U2, T, Td, TT, method, stream=stream, sources=sources)
     e1.record(stream)
     torch.cuda.synchronize()
     L.ast_set_tile_rows(old)
@@ -99,12 +99,16 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
     microbenchmark distribution, 2% near vacuum), FP64 and FP32 (the exact one
     also with the learned newton start): Gfaces/s over `reps` launches with the
     inputs resident, HBM fraction at 104 / 52 B per face, and the exact
-    solvers' iterations per face from one counting launch."""
+    solvers' iterations per face from one counting launch; then the
+    passive-scalar fluxes (the live HLLC's scalar and HLLCQ) in FP64 at 128 B
+    per face."""
     import ctypes
 
     import torch
 
-    from asteriy_2d_amd import EXACT, HLL, HLLC, HLLC_A, facegen, native, solver
+    import numpy as np
+
+    from asteriy_2d_amd import EXACT, HLL, HLLC, HLLC_A, HLLCQ, facegen, native, solver
     dev = torch.device("cuda:0")
     F64 = torch.from_numpy(facegen.faces(n)).to(dev)
     F32 = F64.float()
@@ -154,7 +158,18 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
                             "learned_faces": d["mlp_faces"], "capped": d["capped"]}
             r["iterations"] = its
         res[name] = r
-    del F64, F32, o64, o32
+    del F32, o32
+    # passive-scalar fluxes (10 planes in, 6 out: 128 B per face)
+    rng = np.random.default_rng(5)
+    phi = torch.from_numpy(rng.uniform(0.1, 2.0, (2, n))).to(dev)
+    Fq = torch.cat([F64, (phi[0] * F64[0])[None], (phi[1] * F64[4])[None]]).contiguous()
+    del F64, o64, phi
+    oq = torch.empty((6, n), dtype=torch.float64, device=dev)
+    for name, m in (("hllc_scalar", HLLC), ("hllcq", HLLCQ)):
+        tq = timed(lambda: solver.flux_q_batch(m, Fq, out=oq, stream=stream))
+        res[name] = {"f64_gfaces": round(n / tq / 1e6, 3),
+                     "f64_hbm_frac": round(128.0 * n / (tq * 1e-3) / 8e12, 4)}
+    del Fq, oq
     return res
 
 
@@ -435,6 +450,25 @@ def main(argv=None):
     }
     del r32
     if not a.quick:
+        # the FP64 step with add_MK's neutral-H planes (nn2 triples + nn3:
+        # 32 more bytes per cell read) and with add2's pulsed inflow on the
+        # wind's Dirichlet side, kernel alone as above
+        from asteriy_2d_amd import BC_DIRICHLET_PULSED, PULSE_ADD2, solver
+        rng = np.random.default_rng(9)
+        dev = torch.device("cuda:0")
+        nn2 = torch.from_numpy(rng.uniform(-1e-3, 1e-3, (g.M, g.N, 3))).to(dev)
+        nn3 = torch.from_numpy(rng.uniform(-1e-4, 1e-4, (g.M, g.N))).to(dev)
+        ksrc = _kernel_ms(g, "f64", HLLC, torch.cuda.current_stream(), reps=20,
+                          sources=solver.NeutralSources(nn2, nn3, ken=0.02))
+        del nn2, nn3
+        gp = g.replace(bc=(g.bc[0], BC_DIRICHLET_PULSED, g.bc[2], g.bc[3]), bc_pulse=PULSE_ADD2)
+        kpul = _kernel_ms(gp, "f64", HLLC, torch.cuda.current_stream(), reps=20)
+        extra["fp64_step_variants"] = {
+            "plain_us": round(kms64 * 1e3, 2),
+            "neutral_sources_us": round(ksrc * 1e3, 2),
+            "neutral_sources_hbm_frac": round(96.0 * cells / (ksrc * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pulsed_inflow_us": round(kpul * 1e3, 2)}
+        log(f"FP64 step variants: {extra['fp64_step_variants']}")
         sr = p.s.astype(np.float32).astype(np.float64)
         ur = p.u.astype(np.float32).astype(np.float64)
         _, rfl, _, _, _ = _run_sim(g, HLLC, "f64", sr, ur, K)
@@ -548,18 +582,35 @@ def main(argv=None):
             bt.set_time(k)
         mb = _timed(bt, Kb, bt.torch_stream())
         bcells = bt.cells
-        errs = []
+        errs, fl_init, fl_store = [], [], []
         for k, x in enumerate(inst[:16]):  # per-instance FP64 reference on a subset
             sim = Simulation(x[0], HLLC, "f64", x[1], x[2])
             sim.run(Kb).sync()
-            errs.append(_l1(bt.state(k)[0][..., 0], sim.state()[0][..., 0]))
+            ref = sim.state()
             sim.close()
+            errs.append(_l1(bt.state(k)[0][..., 0], ref[0][..., 0]))
+            # the instance's floors: FP64 from the FP32-rounded initial state,
+            # and FP64 with the state rounded to FP32 after every step
+            sim = Simulation(x[0], HLLC, "f64", x[1].astype(np.float32).astype(np.float64),
+                             x[2].astype(np.float32).astype(np.float64))
+            sim.run(Kb).sync()
+            fl_init.append(_l1(sim.state()[0][..., 0], ref[0][..., 0]))
+            sim.close()
+            fl_store.append(_l1(_fp64_fp32_state(x[0], x[1], x[2], Kb)[0][..., 0], ref[0][..., 0]))
         bt.close()
+        ratio = np.array(errs) / np.maximum(np.array(fl_store), 1e-300)
         extra["batch_64"] = {
             "instances": 64, "steps": Kb, "cells": bcells,
             "gcells": round(bcells * Kb / mb / 1e6, 3),
             "l1_rho_vs_fp64": {"instances_checked": len(errs), "median": float(np.median(errs)),
                                "max": float(np.max(errs))},
+            "floor_fp32_initial_state": {"median": float(np.median(fl_init)),
+                                         "max": float(np.max(fl_init))},
+            "floor_fp32_state_every_step": {"median": float(np.median(fl_store)),
+                                            "max": float(np.max(fl_store))},
+            "error_over_storage_floor": {"median": float(np.median(ratio)),
+                                         "max": float(np.max(ratio)),
+                                         "per_instance": [round(float(r), 3) for r in ratio]},
         }
         log(f"batch: {extra['batch_64']}")
 
