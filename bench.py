@@ -327,13 +327,15 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
-    ap.add_argument("--profile", choices=["f32", "f64", "exact32", "exact32l", "exact64", "batch",
-                                          "flux"],
+    ap.add_argument("--profile", choices=["f32", "f64", "f64src", "f64pulse", "exact32",
+                                          "exact32l", "exact64", "batch", "flux"],
                     default=None,
                     help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
-                         "step in FP32 / FP64 from the initial state, or the exact-solver step "
-                         "(FP32, FP32 with the learned guess, FP64); or --steps steps of the "
-                         "64-instance sweep (batch); or the Riemann-flux microbenchmark (flux)")
+                         "step in FP32 / FP64 from the initial state (f64src: with neutral-H "
+                         "planes, f64pulse: with add2's pulsed inflow on the Dirichlet side), "
+                         "or the exact-solver step (FP32, FP32 with the learned guess, FP64); "
+                         "or --steps steps of the 64-instance sweep (batch); or the "
+                         "Riemann-flux microbenchmark (flux)")
     ap.add_argument("--rows", type=int, default=0,
                     help="--profile: FP32 strip tiling, rows per strip or -S strips per band "
                          "(default: the autotuner's choice)")
@@ -406,13 +408,22 @@ def main(argv=None):
         if a.grid:
             n_, m_ = (int(v) for v in a.grid.lower().split("x"))
             p = problems.wind(n_, m_, method=HLLC, steps=K)
-        sim = Simulation(p.grid, HLLC, a.profile, p.s, p.u)
+        gp = p.grid
+        if a.profile == "f64pulse":
+            from asteriy_2d_amd import BC_DIRICHLET_PULSED, PULSE_ADD2
+            gp = gp.replace(bc=(gp.bc[0], BC_DIRICHLET_PULSED, gp.bc[2], gp.bc[3]),
+                            bc_pulse=PULSE_ADD2)
+        sim = Simulation(gp, HLLC, "f32" if a.profile == "f32" else "f64", p.s, p.u)
+        if a.profile == "f64src":
+            rng = np.random.default_rng(9)
+            sim.set_sources(rng.uniform(-1e-3, 1e-3, (gp.M, gp.N, 3)),
+                            rng.uniform(-1e-4, 1e-4, (gp.M, gp.N)), ken=0.02)
         if a.rows:
             sim.set_tile_rows(a.rows)
         else:
             sim.autotune()
         sim.run(K).sync()
-        log(f"step_{a.profile} HLLC {p.grid.N}x{p.grid.M}: {K} steps")
+        log(f"step_{a.profile} HLLC {gp.N}x{gp.M}: {K} steps")
         sim.close()
         return 0
 
