@@ -303,18 +303,12 @@ AST_DEV int pattern_key(const PrimF& L, const PrimF& R, const ExactCtx& e) {
 // FP64 sound speed of an FP32 state (for the FP64-formed differences)
 AST_DEV double sound_speed_d(double g, float p, float r) { return sqrt(g * (double)p / (double)r); }
 
-// The exact face flux in the face frame (u normal, v tangential); UU is the
-// largest |wave speed| used for dt.  `guess` (finite, > 0) seeds newton.
-// LEARNED (compiled only into the kernels that serve a learned guess; its
-// network needs registers the plain solver does not) with e.mlp set: the
-// caller guarantees a full wave (EXEC all ones); if any lane of the wave
-// holds a strong two-shock face, the wave evaluates the learned guess for all
-// its lanes on MFMA (a wave-uniform branch) and those lanes start newton
-// from it.
 // The exact face flux in the face frame (u normal, v tangential), kernel.cu:
-// 128-570 in FP32 (see the header); UU is the largest |wave speed| used for
-// dt; fi reports the wave pattern, iterations and flags.  `guess` (finite,
-// > 0) seeds newton.
+// 128-570 in FP32 (see the header), as the physical flux of the state it
+// samples at the face, written in that state's frame (FaceF: m = r un,
+// w = (un, ut), gn = p, gt = 0, ge = un p/(g-1)); UU is the largest |wave
+// speed| used for dt; fi reports the wave pattern, iterations and flags.
+// `guess` (finite, > 0) seeds newton.
 // LEARNED (compiled only into the kernels that serve a learned guess; its
 // network needs registers the plain solver does not) with e.mlp set: the
 // caller guarantees a full wave (EXEC all ones); if any lane of the wave
@@ -323,8 +317,8 @@ AST_DEV double sound_speed_d(double g, float p, float r) { return sqrt(g * (doub
 // newton from it.
 // DETECT: report strong two-shock faces (fi.strong) without the network.
 template <bool LEARNED, bool DETECT = false>
-AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
-                        FaceInfo& fi, float guess = -1.f) {
+AST_DEV FaceF exact_frame_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
+                              FaceInfo& fi, float guess = -1.f) {
   const float g = e.g, g1 = e.g1, g2 = e.g2, gm = e.gm;
   // swap sides so that pI <= pII, kernel.cu:156-184
   const bool sw = L.p > R.p;
@@ -483,13 +477,21 @@ AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float
   }
   if (!(UU > 1e-7f)) UU = 1e-7f;  // eps8 floor of the reference's dt, kernel.cu:246-253
   if (sw) en = -en;
-  const float E = pp * e.ig1 + 0.5f * r * (en * en + te * te);
-  FluxF f;
+  FaceF f;
   f.m = r * en;
-  f.un = f.m * en + pp;
-  f.ut = f.m * te;
-  f.e = en * (E + pp);
+  f.wn = en;
+  f.wt = te;
+  f.gn = pp;
+  f.gt = 0.f;
+  f.ge = en * pp * e.ig1;
   return f;
+}
+
+// The same flux in the lab frame (face-frame axes).
+template <bool LEARNED, bool DETECT = false>
+AST_DEV FluxF exact_f32(const PrimF& L, const PrimF& R, const ExactCtx& e, float& UU,
+                        FaceInfo& fi, float guess = -1.f) {
+  return lab_flux(exact_frame_f32<LEARNED, DETECT>(L, R, e, UU, fi, guess));
 }
 
 }  // namespace ast

@@ -183,12 +183,55 @@ constexpr bool is_exact(int method) {
 }
 
 template <int METHOD>
-AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
+AST_DEV FaceF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& UU,
                         const ExactCtx& ex, FaceInfo& fi) {
-  if constexpr (METHOD == AST_HLL) return hll_f32(L, R, c.g, c.ig1, UU);
-  else if constexpr (METHOD == AST_HLLC) return hllc_f32<false>(L, R, c.g, c.ig1, UU);
-  else if constexpr (METHOD == AST_HLLC_A) return hllc_f32<true>(L, R, c.g, c.ig1, UU);
-  else return exact_f32<METHOD == kExactLearned, METHOD == kExactDetect>(L, R, ex, UU, fi);
+  if constexpr (METHOD == AST_HLL) return hll_frame_f32(L, R, c.g, c.ig1, UU);
+  else if constexpr (METHOD == AST_HLLC) return hllc_frame_f32<false>(L, R, c.g, c.ig1, UU);
+  else if constexpr (METHOD == AST_HLLC_A) return hllc_frame_f32<true>(L, R, c.g, c.ig1, UU);
+  else return exact_frame_f32<METHOD == kExactLearned, METHOD == kExactDetect>(L, R, ex, UU, fi);
+}
+
+// Every face of the step is the physical flux of one state (gt = 0) unless
+// HLL may serve it (the method, or the faces outside a high-order window).
+template <int METHOD, bool WINDOW>
+constexpr bool state_faces() {
+  return METHOD != AST_HLL && !WINDOW;
+}
+
+// The flux of the face of lane + 1 (the east face of this lane's cell).
+template <bool STATE>
+AST_DEV FaceF shl1(const FaceF& f) {
+  FaceF r;
+  r.m = shl1(f.m);
+  r.wn = shl1(f.wn);
+  r.wt = shl1(f.wt);
+  r.gn = shl1(f.gn);
+  r.gt = STATE ? 0.f : shl1(f.gt);
+  r.ge = shl1(f.ge);
+  return r;
+}
+
+// A face's flux seen from a cell moving with (cn, ct) (face-frame
+// components): the momentum fluxes gn - m dn, gt - m dt and the energy flux
+// ge - dn gn - dt gt + m |d|^2/2, d = (cn, ct) - w.  The mass flux is
+// frame-free.
+struct CellFlux {
+  float n, t, e;
+};
+template <bool STATE>
+AST_DEV CellFlux in_cell_frame(const FaceF& f, float cn, float ct) {
+  const float dn = cn - f.wn, dt = ct - f.wt;
+  const float hk = 0.5f * f.m * fmaf(dn, dn, dt * dt);
+  CellFlux r;
+  r.n = fmaf(-f.m, dn, f.gn);
+  if constexpr (STATE) {
+    r.t = -f.m * dt;
+    r.e = fmaf(-dn, f.gn, f.ge) + hk;
+  } else {
+    r.t = fmaf(-f.m, dt, f.gt);
+    r.e = fmaf(-dt, f.gt, fmaf(-dn, f.gn, f.ge)) + hk;
+  }
+  return r;
 }
 
 // One face in the x (NORMAL_X) or y direction: rotate to the face frame,
@@ -197,19 +240,19 @@ AST_DEV FluxF face_flux(const PrimF& L, const PrimF& R, const Consts& c, float& 
 // The exact solver's FaceInfo goes to st.add() after the call, in converged
 // code (faces diverge inside with a window).
 template <int METHOD, bool WINDOW, bool NORMAL_X, class ST>
-AST_DEV FluxF face(const Q& l, const Q& r, const Q& lc, const Q& rc, bool ho, const Consts& c,
+AST_DEV FaceF face(const Q& l, const Q& r, const Q& lc, const Q& rc, bool ho, const Consts& c,
                    float& UU, const ExactCtx& ex, ST& st) {
   PrimF L, R;
   FaceInfo fi;
   fi.pat = kNoFace;
   fi.it = 0;
   fi.capped = fi.island = fi.learned = fi.strong = false;
-  FluxF f;
+  FaceF f;
   if constexpr (WINDOW) {  // faces diverge here: no wave-level learned guess
     if (!ho) {
       L = NORMAL_X ? PrimF{lc.r, lc.p, lc.vx, lc.vy} : PrimF{lc.r, lc.p, lc.vy, lc.vx};
       R = NORMAL_X ? PrimF{rc.r, rc.p, rc.vx, rc.vy} : PrimF{rc.r, rc.p, rc.vy, rc.vx};
-      f = hll_f32(L, R, c.g, c.ig1, UU);
+      f = hll_frame_f32(L, R, c.g, c.ig1, UU);
     } else {
       ExactCtx e2 = ex;
       e2.mlp = nullptr;
@@ -361,7 +404,7 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
 
   // prologue: rows m0-2 .. m0+1, the south face of row m0
   Q w[4], nv[2], dd[2];
-  FluxF G[2];
+  FaceF G[2];
   bool fr[2] = {false, false};
   {
     const Q qa = LD(m0 - 2), qb = LD(m0 - 1);
@@ -417,7 +460,7 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     const float y = a.ymin + m * a.dyf;
     const bool live = FAST || m < m1;  // wave-uniform
     float uu;
-    const FluxF Gn = face<METHOD, WINDOW, false>(nv[R::e0], sv1, q0, q1, a.ho_y, c, uu, ex, st);
+    const FaceF Gn = face<METHOD, WINDOW, false>(nv[R::e0], sv1, q0, q1, a.ho_y, c, uu, ex, st);
     nv[R::e1] = nv1;
     // x: face values of row m, its west face (left state = east value of
     // lane-1); the east face flux is the west face flux of lane+1.  The
@@ -429,7 +472,7 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     Q ev, wv;
     face_values(q0, dl, dr, ev, wv);
     float ux;
-    const FluxF Fw = face<METHOD, WINDOW, true>(shr1_keep(ev), wv, ql, q0, a.ho_x, c, ux, ex, st);
+    const FaceF Fw = face<METHOD, WINDOW, true>(shr1_keep(ev), wv, ql, q0, a.ho_x, c, ux, ex, st);
     if (frz) {
       const bool f0 = fr[R::e0];
       const bool f1 = frozen_at(cw, a.x, y + a.dyf);
@@ -441,40 +484,51 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
       UUy = fmaxf(UUy, live ? uu : 0.f);
       UUx = fmaxf(UUx, live ? ux : 0.f);
     }
-    const FluxF& Gs = G[R::e0];
-    // conservative update, kernel.cu:3027-3032, 3059-3073 (x faces: Fe - Fw,
-    // y faces: Gn - Gs), dU = -T_do/dx (Fe - Fw) - T_do/dy (Gn - Gs)
-    float dr_ = a.ax * (shl1(Fw.m) - Fw.m) + a.ay * (Gn.m - Gs.m);
-    float dE = a.ax * (shl1(Fw.e) - Fw.e) + a.ay * (Gn.e - Gs.e);
-    float dmx = a.ax * (shl1(Fw.un) - Fw.un) + a.ay * (Gn.ut - Gs.ut);
-    float dmy = a.ax * (shl1(Fw.ut) - Fw.ut) + a.ay * (Gn.un - Gs.un);
+    const FaceF& Gs = G[R::e0];
+    // conservative update, kernel.cu:3027-3032, 3059-3073, written in the
+    // frame of the cell (velocity u = (vx, vy)): with a = -T_do/dx, -T_do/dy
+    // and each face's flux seen from the cell (in_cell_frame),
+    //   D_rho = a (m_E - m_W) + ...             (rho' = rho + D_rho)
+    //   D_1   = a (g_E - g_W) + ...  = dm - u D_rho
+    //   D_2   = a (e_E - e_W) + ...  = dE - u.dm + |u|^2 D_rho / 2
+    // and, exactly, u' = u + D_1/rho', rho' e' = rho e + D_2 - |D_1|^2/(2 rho').
+    // No term is of the size of the kinetic energy, so the internal energy
+    // -- O(1/M^2) of the total at Mach M -- keeps FP32's relative accuracy.
+    constexpr bool ST1 = state_faces<METHOD, WINDOW>();
+    const FaceF Fe = shl1<ST1>(Fw);
+    const CellFlux cE = in_cell_frame<ST1>(Fe, q0.vx, q0.vy);
+    const CellFlux cW = in_cell_frame<ST1>(Fw, q0.vx, q0.vy);
+    const CellFlux cN = in_cell_frame<ST1>(Gn, q0.vy, q0.vx);
+    const CellFlux cS = in_cell_frame<ST1>(Gs, q0.vy, q0.vx);
+    float dr_ = a.ax * (Fe.m - Fw.m) + a.ay * (Gn.m - Gs.m);
+    float de = a.ax * (cE.e - cW.e) + a.ay * (cN.e - cS.e);
+    const float d1x = a.ax * (cE.n - cW.n) + a.ay * (cN.t - cS.t);
+    const float d1y = a.ax * (cE.t - cW.t) + a.ay * (cN.n - cS.n);
     G[R::e1] = Gn;
-    const float ke = 0.5f * q0.r * (q0.vx * q0.vx + q0.vy * q0.vy);
-    const float mx = q0.r * q0.vx, my = q0.r * q0.vy;
     if constexpr (GEOM == AST_AXISYMMETRIC) {
-      // KorolkovSolver.cu:13-32: -(T_do/y) v {rho, rho u, rho v, E + p}
+      // KorolkovSolver.cu:13-32: -(T_do/y) v {rho, rho u, rho v, E + p}; in
+      // the cell frame the momentum part vanishes and the energy part is
+      // -(T_do/y) v (rho e + p) = -(T_do/y) v gamma p/(gamma-1)
       float ty;
       if constexpr (FAST) ty = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tyl), m - m0));
       else ty = a.tdof * frcp(y);
-      const float tr = ty * my;
-      dr_ -= tr;
-      dmx -= tr * q0.vx;
-      dmy -= tr * q0.vy;
-      dE -= ty * q0.vy * (c.gig1 * q0.p + ke);
+      const float f = ty * q0.vy;
+      dr_ = fmaf(-f, q0.r, dr_);
+      de = fmaf(-f, c.gig1 * q0.p, de);
     }
-    // kernel.cu:3060-3064: a non-positive density keeps the old one; the
-    // density change that enters the kinetic energy below is then zero
+    // kernel.cu:3060-3064: a non-positive density keeps the old one, while
+    // the momentum change stays (the reference divides rho u + dm by the old
+    // density): then D_1 gains u D_rho and D_2 loses |u|^2 D_rho/2
     const bool rfloor = q0.r + dr_ <= 0.f;
-    const float drf = rfloor ? 0.f : dr_;
-    float r2 = q0.r + drf;
+    const float kept = rfloor ? dr_ : 0.f;
+    float r2 = q0.r + (dr_ - kept);
     const float ir = frcp(r2);
-    float vx2 = (mx + dmx) * ir;
-    float vy2 = (my + dmy) * ir;
-    // p' = (E' - rho'|u'|^2/2)(gamma-1), written so that the O(E) terms
-    // cancel exactly: E' - KE' = p/(g-1) + dE - (KE' - KE) with
-    // KE' - KE = (m.dm + |dm|^2/2 - KE*dr) / rho' (dr = rho' - rho).
-    const float dke = (mx * dmx + my * dmy + 0.5f * (dmx * dmx + dmy * dmy) - ke * drf) * ir;
-    float p2 = q0.p + c.g1 * (dE - dke);
+    const float e1x = fmaf(q0.vx, kept, d1x), e1y = fmaf(q0.vy, kept, d1y);
+    float vx2 = fmaf(e1x, ir, q0.vx);
+    float vy2 = fmaf(e1y, ir, q0.vy);
+    const float d2f = de - 0.5f * (fmaf(q0.vx, q0.vx, q0.vy * q0.vy) * kept +
+                                   fmaf(e1x, e1x, e1y * e1y) * ir);
+    float p2 = fmaf(c.g1, d2f, q0.p);
     p2 = (p2 <= 0.f) ? 0.000001f : p2;  // kernel.cu:3070-3073
     if (frz && fr[R::e0]) {  // kernel.cu:2317-2324
       r2 = q0.r;
