@@ -518,16 +518,26 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     }
     // kernel.cu:3060-3064: a non-positive density keeps the old one, while
     // the momentum change stays (the reference divides rho u + dm by the old
-    // density): then D_1 gains u D_rho and D_2 loses |u|^2 D_rho/2
+    // density): then D_1 gains u D_rho and D_2 loses |u|^2 D_rho/2.  Rows
+    // where no lane floors (nearly all) skip those terms (wave-uniform).
     const bool rfloor = q0.r + dr_ <= 0.f;
-    const float kept = rfloor ? dr_ : 0.f;
-    float r2 = q0.r + (dr_ - kept);
-    const float ir = frcp(r2);
-    const float e1x = fmaf(q0.vx, kept, d1x), e1y = fmaf(q0.vy, kept, d1y);
-    float vx2 = fmaf(e1x, ir, q0.vx);
-    float vy2 = fmaf(e1y, ir, q0.vy);
-    const float d2f = de - 0.5f * (fmaf(q0.vx, q0.vx, q0.vy * q0.vy) * kept +
-                                   fmaf(e1x, e1x, e1y * e1y) * ir);
+    float r2, vx2, vy2, d2f;
+    if (__builtin_amdgcn_ballot_w64(rfloor) == 0) {
+      r2 = q0.r + dr_;
+      const float ir = frcp(r2);
+      vx2 = fmaf(d1x, ir, q0.vx);
+      vy2 = fmaf(d1y, ir, q0.vy);
+      d2f = de - 0.5f * fmaf(d1x, d1x, d1y * d1y) * ir;
+    } else {
+      const float kept = rfloor ? dr_ : 0.f;
+      r2 = q0.r + (dr_ - kept);
+      const float ir = frcp(r2);
+      const float e1x = fmaf(q0.vx, kept, d1x), e1y = fmaf(q0.vy, kept, d1y);
+      vx2 = fmaf(e1x, ir, q0.vx);
+      vy2 = fmaf(e1y, ir, q0.vy);
+      d2f = de - 0.5f * (fmaf(q0.vx, q0.vx, q0.vy * q0.vy) * kept +
+                         fmaf(e1x, e1x, e1y * e1y) * ir);
+    }
     float p2 = fmaf(c.g1, d2f, q0.p);
     p2 = (p2 <= 0.f) ? 0.000001f : p2;  // kernel.cu:3070-3073
     if (frz && fr[R::e0]) {  // kernel.cu:2317-2324
