@@ -36,6 +36,7 @@ SOURCES = [
     ("sim.cpp", PARITY),
     ("misc.hip", PARITY),
     ("flux_f64.hip", PARITY),
+    ("flux_mhd_f64.hip", PARITY),
     ("step_f64.hip", PARITY),
     ("flux_f32.hip", FAST),
     ("step_f32.hip", FAST),

@@ -14,6 +14,7 @@ from typing import List, Sequence
 
 HLL, HLLC, EXACT, HLLC_A = 0, 1, 2, 3
 HLLCQ, HLLCQ_A = 4, 5  # passive-scalar fluxes (ast_flux_q_*)
+HLLC_MHD, HLLD = 6, 7  # MHD fluxes with a passive scalar (ast_flux_mhd_f64)
 METHODS = {"hll": HLL, "hllc": HLLC, "exact": EXACT, "hllc_a": HLLC_A}
 PLANAR, AXISYMMETRIC = 0, 1
 BC_OUTFLOW, BC_REFLECT, BC_DIRICHLET, BC_OUTFLOW_CLAMPED, BC_DIRICHLET_PULSED = 0, 1, 2, 3, 4

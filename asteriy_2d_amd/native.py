@@ -37,6 +37,8 @@ SIGNATURES = {
                               _c_float, _vp, _vp, _vp]),
     "ast_flux_q_f64": (_c_int, [_c_int, _c_size, _vp, _c_double, _c_double, _c_double,
                                 _c_double, _c_double, _vp, _vp]),
+    "ast_flux_mhd_f64": (_c_int, [_c_int, _c_size, _vp, _c_double, _c_double, _c_double,
+                                  _c_double, _c_double, _c_double, _vp, _vp]),
     "ast_step_f64": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp]),
     "ast_step_f64_src": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
                                   ctypes.POINTER(CSources), _vp, _vp]),

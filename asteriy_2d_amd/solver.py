@@ -21,7 +21,7 @@ import ctypes
 
 from . import native
 from .grid import EXACT, HLL as M_HLL, HLLC as M_HLLC, HLLC_A as M_HLLC_A
-from .grid import HLLCQ as M_HLLCQ, HLLCQ_A as M_HLLCQ_A
+from .grid import HLLCQ as M_HLLCQ, HLLCQ_A as M_HLLCQ_A, HLLC_MHD as M_HLLC_MHD, HLLD as M_HLLD
 
 _STATS_WORDS = 10
 
@@ -196,6 +196,32 @@ def HLLCQ_Aleksashov(faces, n1, n2, rad, **kw):
     """Gas_Dinamic.cu:355 (zero magnetic field) over a batch of faces with a
     passive scalar."""
     return flux_q_batch(M_HLLCQ_A, faces, n1, n2, rad, **kw)
+
+
+def flux_mhd_batch(method: int, faces, n1=1.0, n2=0.0, n3=0.0, rad=1.0, gamma=5.0 / 3.0,
+                   krit=0.05, out=None, stream=None):
+    """The MHD operators over n faces: faces is a (18, n) float64 device
+    tensor (rho, Q, p, v1, v2, v3, Bx, By, Bz of the left state, then of the
+    right); returns (10, n): mass, momentum x/y/z, field x/y/z, energy
+    fluxes, the scalar's flux PQ and dt.  method: HLLC_MHD or HLLD."""
+    import torch
+    if (faces.dim() != 2 or faces.shape[0] != 18 or not faces.is_contiguous()
+            or faces.dtype != torch.float64):
+        raise ValueError("faces must be a contiguous (18, n) float64 tensor")
+    n = faces.shape[1]
+    if out is None:
+        out = torch.empty((10, n), dtype=faces.dtype, device=faces.device)
+    native.call("ast_flux_mhd_f64", int(method), n, _ptr(faces), float(n1), float(n2),
+                float(n3), float(rad), float(gamma), float(krit), _ptr(out), _stream(stream))
+    return out
+
+
+def HLLDQ_Korolkov(faces, n1, n2, n3, rad, metod, **kw):
+    """Gas_Dinamic.cu:874 over a batch of faces: metod 2 is HLLC for MHD,
+    metod 3 is HLLD (the reference's numbering)."""
+    if metod not in (2, 3):
+        raise ValueError("metod must be 2 (HLLC for MHD) or 3 (HLLD)")
+    return flux_mhd_batch(M_HLLD if metod == 3 else M_HLLC_MHD, faces, n1, n2, n3, rad, **kw)
 
 
 def conservation_sums(grid, s, u, stream=None):

@@ -101,7 +101,8 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
     inputs resident, HBM fraction at 104 / 52 B per face, and the exact
     solvers' iterations per face from one counting launch; then the
     passive-scalar fluxes (the live HLLC's scalar and HLLCQ) in FP64 at 128 B
-    per face."""
+    per face, and the MHD fluxes (HLLC for MHD, HLLD) in FP64 at 224 B per
+    face."""
     import ctypes
 
     import torch
@@ -170,6 +171,58 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
         res[name] = {"f64_gfaces": round(n / tq / 1e6, 3),
                      "f64_hbm_frac": round(128.0 * n / (tq * 1e-3) / 8e12, 4)}
     del Fq, oq
+    res.update(_mhd_microbenchmark(n, reps))
+    return res
+
+
+def _mhd_microbenchmark(n: int = 1 << 24, reps: int = 5):
+    """The MHD fluxes with a scalar (HLLC for MHD, HLLD) in FP64 over n seeded
+    face pairs with the inputs resident: Gfaces/s over `reps` launches and
+    the HBM fraction at 224 B per face (18 planes in, 10 out)."""
+    import torch
+
+    from asteriy_2d_amd import HLLC_MHD, HLLD, solver
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps  # ms per launch
+
+    res = {}
+    # seeded states over the same decades of rho and p, Mach up to ~5, plasma beta
+    # ~1e-3..1e3, a continuous normal field
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+
+    def unif(lo, hi):
+        return lo + (hi - lo) * torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+
+    def normal3():
+        return torch.randn((3, n), generator=gen, device=dev, dtype=torch.float64)
+
+    planes = []
+    for _ in range(2):
+        r, pr = torch.exp(unif(-3.0, 3.0)), torch.exp(unif(-4.0, 3.0))
+        v = normal3() * 1.5 * torch.sqrt(5.0 / 3.0 * pr / r)
+        b = normal3() * torch.sqrt(pr) * torch.exp(unif(-1.5, 1.5)) * 3.544907701811032
+        planes += [r, unif(-2.0, 2.0) * r, pr, v[0], v[1], v[2], b[0], b[1], b[2]]
+    Fm = torch.stack(planes).contiguous()
+    del planes
+    Fm[15] = Fm[6]
+    om = torch.empty((10, n), dtype=torch.float64, device=dev)
+    for name, m in (("hllc_mhd", HLLC_MHD), ("hlld", HLLD)):
+        tm = timed(lambda: solver.flux_mhd_batch(m, Fm, out=om, stream=stream))
+        res[name] = {"f64_gfaces": round(n / tm / 1e6, 3),
+                     "f64_hbm_frac": round(224.0 * n / (tm * 1e-3) / 8e12, 4)}
+    del Fm, om
     return res
 
 
@@ -341,14 +394,14 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
     ap.add_argument("--profile", choices=["f32", "f64", "f64src", "f64pulse", "exact32",
-                                          "exact32l", "exact64", "batch", "flux"],
+                                          "exact32l", "exact64", "batch", "flux", "mhd"],
                     default=None,
                     help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
                          "step in FP32 / FP64 from the initial state (f64src: with neutral-H "
                          "planes, f64pulse: with add2's pulsed inflow on the Dirichlet side), "
                          "or the exact-solver step (FP32, FP32 with the learned guess, FP64); "
                          "or --steps steps of the 64-instance sweep (batch); or the "
-                         "Riemann-flux microbenchmark (flux)")
+                         "Riemann-flux microbenchmark (flux), or its MHD fluxes alone (mhd)")
     ap.add_argument("--rows", type=int, default=0,
                     help="--profile: FP32 strip tiling, rows per strip or -S strips per band "
                          "(default: the autotuner's choice)")
@@ -398,6 +451,9 @@ def main(argv=None):
     if a.profile == "flux":
         from asteriy_2d_amd import learned
         log(f"flux: {_flux_microbenchmark(learned.LearnedGuess.load())}")
+        return 0
+    if a.profile == "mhd":
+        log(f"mhd: {_mhd_microbenchmark(reps=K)}")
         return 0
     if a.profile in ("exact32", "exact32l", "exact64"):
         f32 = a.profile != "exact64"

@@ -45,6 +45,9 @@ enum { AST_HLL = 0, AST_HLLC = 1, AST_EXACT = 2, AST_HLLC_A = 3 };
  * (Gas_Dinamic.cu:249) and HLLCQ_Aleksashov (Gas_Dinamic.cu:355, zero
  * magnetic field) */
 enum { AST_HLLCQ = 4, AST_HLLCQ_A = 5 };
+/* MHD fluxes with a passive scalar (ast_flux_mhd_f64): HLLDQ_Korolkov
+ * (Gas_Dinamic.cu:874) with metod 2 (HLLC for MHD) and metod 3 (HLLD) */
+enum { AST_HLLC_MHD = 6, AST_HLLD = 7 };
 
 /* ---- geometry of the conservative update ------------------------------- */
 enum {
@@ -158,6 +161,26 @@ int ast_flux_f32(int method, size_t n, const float* in, float n1, float n2,
  * same as HLLC_Aleksashov, AST_HLLC_A). */
 int ast_flux_q_f64(int method, size_t n, const double* in, double n1, double n2,
                    double rad, double gamma, double krit, double* out, void* stream);
+
+/* MHD fluxes with a passive scalar, replacing
+ *   double HLLDQ_Korolkov(ro_L, Q_L, p_L, v1_L, v2_L, v3_L, Bx_L, By_L, Bz_L,
+ *       ro_R, Q_R, p_R, v1_R, v2_R, v3_R, Bx_R, By_R, Bz_R, double* P,
+ *       double& PQ, n1, n2, n3, double& rad, int metod)
+ *   Gas_Dinamic.cu:874-877 (metod 2: AST_HLLC_MHD, metod 3: AST_HLLD)
+ * over n faces with a common normal (n1, n2, n3), gamma for both the sound
+ * speed and the energy (the reference's ggg and ga), fields in the
+ * reference's units (b = B / sqrt(4 pi) inside the solver).
+ * in  : 18 planes of n values: rho, Q, p, v1, v2, v3, Bx, By, Bz of the
+ *       left state, then the same 9 of the right state.
+ * out : 10 planes: P[0..7] (mass; momentum x, y, z; field x, y, z; energy),
+ *       PQ, and the operator's return value krit*rad/max(|SL|,|SR|).
+ * The frame follows the reference: a normal component above 0.1 (then
+ * below -0.1), tested in the order n1, n2, n3, picks the tangent pair; a
+ * normal that picks none, which the reference reports as an error and then
+ * solves with zero tangents, is refused here (AST_ERR_ARG). */
+int ast_flux_mhd_f64(int method, size_t n, const double* in, double n1, double n2,
+                     double n3, double rad, double gamma, double krit, double* out,
+                     void* stream);
 
 /* ---- the step ------------------------------------------------------------
  * Replaces

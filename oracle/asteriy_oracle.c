@@ -30,6 +30,14 @@
  *                                 ora_hllc to round-off where both sides have
  *                                 the same sound speed (the two wave-speed
  *                                 estimates then coincide).
+ *  ora_hlldq (HLLC-MHD, HLLD)   : consistency with the physical MHD flux,
+ *                                 mirror antisymmetry on the six axis
+ *                                 normals, exact resolution of isolated
+ *                                 contacts (both), tangential and rotational
+ *                                 discontinuities (HLLD), Toro's HLLC at
+ *                                 B = 0, the integral consistency of the fan
+ *                                 with the HLL average, the fast speed's
+ *                                 closed forms in the returned dt.
  *  ora_godunov (+newton/devtwo) : independent Toro exact solver (toro.c),
  *                                 Toro's published test table, the
  *                                 reference's own `test` kernel KAT
@@ -551,6 +559,303 @@ void ora_flux_q_batch(int method, double gamma, double krit, long n,
     out[3 * n + i] = Pu[1];
     out[4 * n + i] = t;
     out[5 * n + i] = PQ;
+  }
+}
+
+/* ---- MHD fluxes with a passive scalar: HLLDQ_Korolkov ----------------- */
+
+/* spi4, Header.h:69: sqrt(4 pi) as the reference writes it. */
+#define O_SPI4 3.544907701811032
+
+/* HLLDQ_Korolkov, Gas_Dinamic.cu:874-1415, metod 2 (HLLC for MHD: one star
+ * state per side, tangential velocity and field from the HLL average,
+ * Gas_Dinamic.cu:1040-1210) and metod 3 (HLLD: the outer star states and the
+ * Alfven-wave inner states, Gas_Dinamic.cu:1211-1410).
+ * L, R: rho, Q, p, v1, v2, v3, Bx, By, Bz (the reference's argument order).
+ * P: the 8 fluxes in the reference's output order (mass, momentum x/y/z,
+ * field x/y/z, energy: the swap at Gas_Dinamic.cu:1204-1208); PQ the
+ * scalar's flux; returns krit*rad/max(|SL|,|SR|).
+ * The frame (n, t, m) follows Gas_Dinamic.cu:894-931 (a component above
+ * 0.1 / below -0.1 picks it); for a normal that picks none the reference
+ * prints and goes on with t = m = 0, as here.  Values the reference computes
+ * and never uses (aL, aR, ca, UZ4, rzL, rzR) are left out.  Where no branch
+ * of the flux selection matches (NaN speeds) the reference leaves PO
+ * uninitialised; here the fluxes are NaN.  `fan`, when given, receives the
+ * wave speeds and the normal-frame states of the fan (test access). */
+static double o_hlldq(const ora_phys* ph, const double* Lv, const double* Rv,
+                      double n1, double n2, double n3, double rad, int metod,
+                      double* P, double* PQ, ora_mhd_fan* fan) {
+  const double ggg = ph->ga, g1 = ph->g1;
+  const double ro_L = Lv[0], Q_L = Lv[1], p_L = Lv[2];
+  const double v1_L = Lv[3], v2_L = Lv[4], v3_L = Lv[5];
+  const double Bx_L = Lv[6], By_L = Lv[7], Bz_L = Lv[8];
+  const double ro_R = Rv[0], Q_R = Rv[1], p_R = Rv[2];
+  const double v1_R = Rv[3], v2_R = Rv[4], v3_R = Rv[5];
+  const double Bx_R = Rv[6], By_R = Rv[7], Bz_R = Rv[8];
+  /* Gas_Dinamic.cu:879-885 */
+  const double bx_L = Bx_L / O_SPI4, by_L = By_L / O_SPI4, bz_L = Bz_L / O_SPI4;
+  const double bx_R = Bx_R / O_SPI4, by_R = By_R / O_SPI4, bz_R = Bz_R / O_SPI4;
+  /* Gas_Dinamic.cu:887-931 */
+  double t1 = 0.0, t2 = 0.0, t3 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  if (n1 > 0.1) { t2 = 1.0; m3 = 1.0; }
+  else if (n2 > 0.1) { t3 = 1.0; m1 = 1.0; }
+  else if (n3 > 0.1) { t1 = 1.0; m2 = 1.0; }
+  else if (n1 < -0.1) { t3 = -1.0; m2 = -1.0; }
+  else if (n2 < -0.1) { t1 = -1.0; m3 = -1.0; }
+  else if (n3 < -0.1) { t1 = -1.0; m2 = -1.0; }
+  /* Gas_Dinamic.cu:934-948 */
+  const double u1 = v1_L * n1 + v2_L * n2 + v3_L * n3;
+  const double v1 = v1_L * t1 + v2_L * t2 + v3_L * t3;
+  const double w1 = v1_L * m1 + v2_L * m2 + v3_L * m3;
+  const double u2 = v1_R * n1 + v2_R * n2 + v3_R * n3;
+  const double v2 = v1_R * t1 + v2_R * t2 + v3_R * t3;
+  const double w2 = v1_R * m1 + v2_R * m2 + v3_R * m3;
+  const double bn1 = bx_L * n1 + by_L * n2 + bz_L * n3;
+  const double bt1 = bx_L * t1 + by_L * t2 + bz_L * t3;
+  const double bm1 = bx_L * m1 + by_L * m2 + bz_L * m3;
+  const double bn2 = bx_R * n1 + by_R * n2 + bz_R * n3;
+  const double bt2 = bx_R * t1 + by_R * t2 + bz_R * t3;
+  const double bm2 = bx_R * m1 + by_R * m2 + bz_R * m3;
+  /* Gas_Dinamic.cu:952-973 */
+  const double bb_L = bx_L * bx_L + by_L * by_L + bz_L * bz_L;
+  const double bb_R = bx_R * bx_R + by_R * by_R + bz_R * bz_R;
+  const double uu_L = (v1_L * v1_L + v2_L * v2_L + v3_L * v3_L) / 2.0;
+  const double uu_R = (v1_R * v1_R + v2_R * v2_R + v3_R * v3_R) / 2.0;
+  const double cfL = sqrt((ggg * p_L + bb_L +
+                           sqrt((ggg * p_L + bb_L) * (ggg * p_L + bb_L) -
+                                4.0 * ggg * p_L * (bn1 * bn1))) / (2.0 * ro_L));
+  const double cfR = sqrt((ggg * p_R + bb_R +
+                           sqrt((ggg * p_R + bb_R) * (ggg * p_R + bb_R) -
+                                4.0 * ggg * p_R * (bn2 * bn2))) / (2.0 * ro_R));
+  const double SL = fmin(u1, u2) - fmax(cfL, cfR);
+  const double SR = fmax(u1, u2) + fmax(cfL, cfR);
+  /* Gas_Dinamic.cu:976-990 */
+  const double pTL = p_L + bb_L / 2.0;
+  const double pTR = p_R + bb_R / 2.0;
+  const double suR = (SR - u2);
+  const double suL = (SL - u1);
+  const double SM = (suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL) /
+                    (suR * ro_R - suL * ro_L);
+  const double PTT = (suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1)) /
+                     (suR * ro_R - suL * ro_L);
+  const double UU = fmax(fabs(SL), fabs(SR));
+  const double time = ph->krit * rad / UU;
+  /* Gas_Dinamic.cu:992-1037 */
+  double FL[9], FR[9], UL[9], UR[9];
+  const double e1 = p_L / g1 + ro_L * uu_L + bb_L / 2.0;
+  const double e2 = p_R / g1 + ro_R * uu_R + bb_R / 2.0;
+  FL[0] = ro_L * u1;
+  FL[1] = ro_L * u1 * u1 + pTL - bn1 * bn1;
+  FL[2] = ro_L * u1 * v1 - bn1 * bt1;
+  FL[3] = ro_L * u1 * w1 - bn1 * bm1;
+  FL[4] = (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1);
+  FL[5] = 0.0;
+  FL[6] = u1 * bt1 - v1 * bn1;
+  FL[7] = u1 * bm1 - w1 * bn1;
+  FL[8] = Q_L * u1;
+  FR[0] = ro_R * u2;
+  FR[1] = ro_R * u2 * u2 + pTR - bn2 * bn2;
+  FR[2] = ro_R * u2 * v2 - bn2 * bt2;
+  FR[3] = ro_R * u2 * w2 - bn2 * bm2;
+  FR[4] = (e2 + pTR) * u2 - bn2 * (u2 * bn2 + v2 * bt2 + w2 * bm2);
+  FR[5] = 0.0;
+  FR[6] = u2 * bt2 - v2 * bn2;
+  FR[7] = u2 * bm2 - w2 * bn2;
+  FR[8] = Q_R * u2;
+  UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1; UL[4] = e1;
+  UL[5] = bn1; UL[6] = bt1; UL[7] = bm1; UL[8] = Q_L;
+  UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2; UR[4] = e2;
+  UR[5] = bn2; UR[6] = bt2; UR[7] = bm2; UR[8] = Q_R;
+  /* Gas_Dinamic.cu:1039-1047 */
+  const double bn = (SR * UR[5] - SL * UL[5] + FL[5] - FR[5]) / (SR - SL);
+  const double bt = (SR * UR[6] - SL * UL[6] + FL[6] - FR[6]) / (SR - SL);
+  const double bm = (SR * UR[7] - SL * UL[7] + FL[7] - FR[7]) / (SR - SL);
+  const double bbn = bn * bn;
+  const double ro_LL = ro_L * (SL - u1) / (SL - SM);
+  const double ro_RR = ro_R * (SR - u2) / (SR - SM);
+  const double Q_LL = Q_L * (SL - u1) / (SL - SM);
+  const double Q_RR = Q_R * (SR - u2) / (SR - SM);
+
+  double ULL[9], URR[9], ULLL[9], URRR[9], PO[9];
+  double SLL = SM, SRR = SM;
+  int branch = -1;
+  if (metod == 2) {
+    /* Gas_Dinamic.cu:1051-1088 */
+    const double sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1;
+    const double sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2;
+    const double UZ0 = (SR * UR[0] - SL * UL[0] + FL[0] - FR[0]) / (SR - SL);
+    const double UZ1 = (SR * UR[1] - SL * UL[1] + FL[1] - FR[1]) / (SR - SL);
+    const double UZ2 = (SR * UR[2] - SL * UL[2] + FL[2] - FR[2]) / (SR - SL);
+    const double UZ3 = (SR * UR[3] - SL * UL[3] + FL[3] - FR[3]) / (SR - SL);
+    const double suRm = suR / (SR - SM);
+    const double suLm = suL / (SL - SM);
+    const double ptzR = pTR + ro_R * suR * (SM - u2);
+    const double ptzL = pTL + ro_L * suL * (SM - u1);
+    const double ptz = (ptzR + ptzL) / 2.0;
+    const double vRR = UZ2 / UZ0, wRR = UZ3 / UZ0;
+    const double vLL = vRR, wLL = wRR;
+    const double btt2 = bt, bmm2 = bm, btt1 = btt2, bmm1 = bmm2;
+    const double sbvz = (bn * UZ1 + bt * UZ2 + bm * UZ3) / UZ0;
+    const double ee2 = e2 * suRm + (ptz * SM - pTR * u2 + bn * (sbv2 - sbvz)) / (SR - SM);
+    const double ee1 = e1 * suLm + (ptz * SM - pTL * u1 + bn * (sbv1 - sbvz)) / (SL - SM);
+    /* Gas_Dinamic.cu:1138-1158 */
+    ULL[0] = ro_LL; ULL[1] = ro_LL * SM; ULL[2] = ro_LL * vLL; ULL[3] = ro_LL * wLL;
+    ULL[4] = ee1; ULL[5] = bn; ULL[6] = btt1; ULL[7] = bmm1; ULL[8] = Q_LL;
+    URR[0] = ro_RR; URR[1] = ro_RR * SM; URR[2] = ro_RR * vRR; URR[3] = ro_RR * wRR;
+    URR[4] = ee2; URR[5] = bn; URR[6] = btt2; URR[7] = bmm2; URR[8] = Q_RR;
+    for (int i = 0; i < 9; i++) { ULLL[i] = ULL[i]; URRR[i] = URR[i]; }
+    /* Gas_Dinamic.cu:1160-1187 */
+    if (SL >= 0.0) {
+      branch = 0;
+      for (int i = 0; i < 9; i++) PO[i] = FL[i];
+    } else if (SL < 0.0 && SM >= 0.0) {
+      branch = 1;
+      for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i];
+    } else if (SR > 0.0 && SM < 0.0) {
+      branch = 4;
+      for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i];
+    } else if (SR <= 0.0) {
+      branch = 5;
+      for (int i = 0; i < 9; i++) PO[i] = FR[i];
+    }
+  } else {
+    /* metod 3, Gas_Dinamic.cu:1214-1252 */
+    const double ttL = ro_L * suL * (SL - SM) - bbn;
+    const double ttR = ro_R * suR * (SR - SM) - bbn;
+    double vLL, wLL, vRR, wRR, btt1, bmm1, btt2, bmm2;
+    if (fabs(ttL) >= 0.000001) {
+      vLL = v1 - bn * bt1 * (SM - u1) / ttL;
+      wLL = w1 - bn * bm1 * (SM - u1) / ttL;
+      btt1 = bt1 * (ro_L * suL * suL - bbn) / ttL;
+      bmm1 = bm1 * (ro_L * suL * suL - bbn) / ttL;
+    } else {
+      vLL = v1; wLL = w1; btt1 = 0.0; bmm1 = 0.0;
+    }
+    if (fabs(ttR) >= 0.000001) {
+      vRR = v2 - bn * bt2 * (SM - u2) / ttR;
+      wRR = w2 - bn * bm2 * (SM - u2) / ttR;
+      btt2 = bt2 * (ro_R * suR * suR - bbn) / ttR;
+      bmm2 = bm2 * (ro_R * suR * suR - bbn) / ttR;
+    } else {
+      vRR = v2; wRR = w2; btt2 = 0.0; bmm2 = 0.0;
+    }
+    /* Gas_Dinamic.cu:1254-1259 */
+    const double eLL = (e1 * suL + PTT * SM - pTL * u1 +
+                        bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) /
+                       (SL - SM);
+    const double eRR = (e2 * suR + PTT * SM - pTR * u2 +
+                        bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) /
+                       (SR - SM);
+    /* Gas_Dinamic.cu:1261-1296 */
+    const double sqrtroLL = sqrt(ro_LL);
+    const double sqrtroRR = sqrt(ro_RR);
+    SLL = SM - fabs(bn) / sqrtroLL;
+    SRR = SM + fabs(bn) / sqrtroRR;
+    double idbn;
+    if (fabs(bn) > 0.001) {
+      idbn = 1.0 * o_sign(bn);
+    } else {
+      idbn = 0.0;
+      SLL = SM;
+      SRR = SM;
+    }
+    const double vLLL = (sqrtroLL * vLL + sqrtroRR * vRR + idbn * (btt2 - btt1)) / (sqrtroLL + sqrtroRR);
+    const double wLLL = (sqrtroLL * wLL + sqrtroRR * wRR + idbn * (bmm2 - bmm1)) / (sqrtroLL + sqrtroRR);
+    const double bttt = (sqrtroLL * btt2 + sqrtroRR * btt1 + idbn * sqrtroLL * sqrtroRR * (vRR - vLL)) /
+                        (sqrtroLL + sqrtroRR);
+    const double bmmm = (sqrtroLL * bmm2 + sqrtroRR * bmm1 + idbn * sqrtroLL * sqrtroRR * (wRR - wLL)) /
+                        (sqrtroLL + sqrtroRR);
+    const double eLLL = eLL - idbn * sqrtroLL * ((SM * bn + vLL * btt1 + wLL * bmm1) -
+                                                 (SM * bn + vLLL * bttt + wLLL * bmmm));
+    const double eRRR = eRR + idbn * sqrtroRR * ((SM * bn + vRR * btt2 + wRR * bmm2) -
+                                                 (SM * bn + vLLL * bttt + wLLL * bmmm));
+    /* Gas_Dinamic.cu:1300-1342 */
+    ULL[0] = ro_LL; ULL[1] = ro_LL * SM; ULL[2] = ro_LL * vLL; ULL[3] = ro_LL * wLL;
+    ULL[4] = eLL; ULL[5] = bn; ULL[6] = btt1; ULL[7] = bmm1; ULL[8] = Q_LL;
+    URR[0] = ro_RR; URR[1] = ro_RR * SM; URR[2] = ro_RR * vRR; URR[3] = ro_RR * wRR;
+    URR[4] = eRR; URR[5] = bn; URR[6] = btt2; URR[7] = bmm2; URR[8] = Q_RR;
+    ULLL[0] = ro_LL; ULLL[1] = ro_LL * SM; ULLL[2] = ro_LL * vLLL; ULLL[3] = ro_LL * wLLL;
+    ULLL[4] = eLLL; ULLL[5] = bn; ULLL[6] = bttt; ULLL[7] = bmmm; ULLL[8] = Q_LL;
+    URRR[0] = ro_RR; URRR[1] = ro_RR * SM; URRR[2] = ro_RR * vLLL; URRR[3] = ro_RR * wLLL;
+    URRR[4] = eRRR; URRR[5] = bn; URRR[6] = bttt; URRR[7] = bmmm; URRR[8] = Q_RR;
+    /* Gas_Dinamic.cu:1346-1393 */
+    if (SL >= 0.0) {
+      branch = 0;
+      for (int i = 0; i < 9; i++) PO[i] = FL[i];
+    } else if (SL < 0.0 && SLL >= 0.0) {
+      branch = 1;
+      for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i];
+    } else if (SLL <= 0.0 && SM >= 0.0) {
+      branch = 2;
+      for (int i = 0; i < 9; i++)
+        PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i];
+    } else if (SM < 0.0 && SRR > 0.0) {
+      branch = 3;
+      for (int i = 0; i < 9; i++)
+        PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i];
+    } else if (SR > 0.0 && SRR <= 0.0) {
+      branch = 4;
+      for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i];
+    } else if (SR <= 0.0) {
+      branch = 5;
+      for (int i = 0; i < 9; i++) PO[i] = FR[i];
+    }
+  }
+  if (branch < 0)
+    for (int i = 0; i < 9; i++) PO[i] = NAN;
+  if (fan) {
+    fan->S[0] = SL; fan->S[1] = SLL; fan->S[2] = SM; fan->S[3] = SRR; fan->S[4] = SR;
+    fan->branch = branch;
+    for (int i = 0; i < 9; i++) {
+      fan->U[0][i] = UL[i]; fan->U[1][i] = ULL[i]; fan->U[2][i] = ULLL[i];
+      fan->U[3][i] = URRR[i]; fan->U[4][i] = URR[i]; fan->U[5][i] = UR[i];
+      fan->F[0][i] = FL[i]; fan->F[1][i] = FR[i];
+    }
+  }
+  /* Gas_Dinamic.cu:1191-1208 (1397-1413): the normal-field flux, the frame
+   * back to x, y, z, and the energy moved to the last place */
+  const double SN = fmax(fabs(SL), fabs(SR));
+  PO[5] = -SN * (bn2 - bn1);
+  P[0] = PO[0];
+  P[1] = n1 * PO[1] + t1 * PO[2] + m1 * PO[3];
+  P[2] = n2 * PO[1] + t2 * PO[2] + m2 * PO[3];
+  P[3] = n3 * PO[1] + t3 * PO[2] + m3 * PO[3];
+  P[4] = O_SPI4 * (n1 * PO[5] + t1 * PO[6] + m1 * PO[7]);
+  P[5] = O_SPI4 * (n2 * PO[5] + t2 * PO[6] + m2 * PO[7]);
+  P[6] = O_SPI4 * (n3 * PO[5] + t3 * PO[6] + m3 * PO[7]);
+  P[7] = PO[4];
+  *PQ = PO[8];
+  return time;
+}
+
+double ora_hlldq(const ora_phys* ph, const double* L, const double* R, double n1,
+                 double n2, double n3, double rad, int method, double* P,
+                 double* PQ, ora_mhd_fan* fan) {
+  return o_hlldq(ph, L, R, n1, n2, n3, rad, method == AST_HLLD ? 3 : 2, P, PQ, fan);
+}
+
+/* Batched MHD faces: in = 18 planes (rho, Q, p, v1, v2, v3, Bx, By, Bz of
+ * the left state, then of the right), out = 10 planes (the 8 fluxes of
+ * ora_hlldq, PQ, dt).  method: AST_HLLC_MHD or AST_HLLD. */
+void ora_flux_mhd_batch(int method, double gamma, double krit, long n,
+                        const double* in, double n1, double n2, double n3,
+                        double rad, double* out, int nthreads) {
+  ora_phys ph;
+  ora_phys_init(&ph, gamma, krit);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (long i = 0; i < n; i++) {
+    double L[9], R[9], P[8], PQ;
+    for (int k = 0; k < 9; k++) {
+      L[k] = in[k * n + i];
+      R[k] = in[(9 + k) * n + i];
+    }
+    const double t = ora_hlldq(&ph, L, R, n1, n2, n3, rad, method, P, &PQ, NULL);
+    for (int k = 0; k < 8; k++) out[k * n + i] = P[k];
+    out[8 * n + i] = PQ;
+    out[9 * n + i] = t;
   }
 }
 

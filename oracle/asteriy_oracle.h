@@ -50,6 +50,24 @@ double ora_hllcq_a(const ora_phys*, const double* Ls, const double* Lu,
 void ora_flux_q_batch(int method, double gamma, double krit, long n,
                       const double* in, double n1, double n2, double rad,
                       double* out, int nthreads);
+/* MHD fluxes with a passive scalar (HLLDQ_Korolkov, Gas_Dinamic.cu:874):
+ * L, R = rho, Q, p, v1, v2, v3, Bx, By, Bz; P = mass, momentum x/y/z,
+ * field x/y/z, energy; the fan in the frame of the normal: S = SL, SLL, SM,
+ * SRR, SR; U = UL, U*L, U**L, U**R, U*R, UR; F = FL, FR (9 components: rho,
+ * rho u_n, rho u_t, rho u_m, e, b_n, b_t, b_m, Q with b = B/sqrt(4 pi));
+ * branch 0..5 = the fan region of the face, -1 none */
+typedef struct {
+  double S[5];
+  double U[6][9];
+  double F[2][9];
+  int branch;
+} ora_mhd_fan;
+double ora_hlldq(const ora_phys*, const double* L, const double* R, double n1,
+                 double n2, double n3, double rad, int method, double* P,
+                 double* PQ, ora_mhd_fan* fan);
+void ora_flux_mhd_batch(int method, double gamma, double krit, long n,
+                        const double* in, double n1, double n2, double n3,
+                        double rad, double* out, int nthreads);
 double ora_godunov(const ora_phys*, const double* Ls, const double* Lu,
                    const double* Rs, const double* Ru, double n1, double n2,
                    double* Ps, double* Pu, double rad);

@@ -32,6 +32,11 @@ class CInfo(ctypes.Structure):
                                         "enII", "pII", "rII")]
 
 
+class CMhdFan(ctypes.Structure):
+    _fields_ = [("S", ctypes.c_double * 5), ("U", (ctypes.c_double * 9) * 6),
+                ("F", (ctypes.c_double * 9) * 2), ("branch", ctypes.c_int)]
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(LIB_PATH):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -102,6 +107,13 @@ def lib():
         L.ora_flux_q_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_long, _dp, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, _dp, ctypes.c_int]
+        L.ora_hlldq.restype = ctypes.c_double
+        L.ora_hlldq.argtypes = [ctypes.POINTER(CPhys), _dp, _dp, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                _dp, _dp, ctypes.POINTER(CMhdFan)]
+        L.ora_flux_mhd_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_long, _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
         L.ora_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
                                ctypes.c_int, ctypes.c_int]
         L.ora_step_src.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -183,6 +195,39 @@ def flux_q_batch(method: int, faces: np.ndarray, n1=1.0, n2=0.0, rad=1.0, gamma=
     n = faces.shape[1]
     out = np.empty((6, n))
     lib().ora_flux_q_batch(method, gamma, krit, n, _p(faces), n1, n2, rad, _p(out), nthreads)
+    return out
+
+
+def flux_mhd(method: int, L, R, n=(1.0, 0.0, 0.0), rad=1.0, gamma=5.0 / 3.0, krit=0.05,
+             fan: bool = False):
+    """One MHD face (HLLDQ_Korolkov): L, R = (rho, Q, p, v1, v2, v3, Bx, By, Bz).
+    Returns (P (8,), PQ, dt) or, with fan, (P, PQ, dt, fan) where fan is a
+    dict: S (SL, SLL, SM, SRR, SR), U (6, 9) = UL, U*L, U**L, U**R, U*R, UR
+    and F (2, 9) = FL, FR in the frame of the normal, branch (0..5)."""
+    ph = phys(gamma, krit)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (L, R)]
+    assert a[0].shape == (9,) and a[1].shape == (9,)
+    P, PQ = np.zeros(8), np.zeros(1)
+    f = CMhdFan()
+    t = lib().ora_hlldq(ctypes.byref(ph), _p(a[0]), _p(a[1]), float(n[0]), float(n[1]),
+                        float(n[2]), rad, int(method), _p(P), _p(PQ), ctypes.byref(f))
+    if not fan:
+        return P, float(PQ[0]), t
+    d = {"S": np.array(f.S[:]), "U": np.array([f.U[k][:] for k in range(6)]),
+         "F": np.array([f.F[k][:] for k in range(2)]), "branch": int(f.branch)}
+    return P, float(PQ[0]), t, d
+
+
+def flux_mhd_batch(method: int, faces: np.ndarray, n=(1.0, 0.0, 0.0), rad=1.0,
+                   gamma=5.0 / 3.0, krit=0.05, nthreads: int = 0):
+    """faces: (18, n) SoA (rho, Q, p, v1, v2, v3, Bx, By, Bz left, then right)
+    -> out (10, n): the 8 fluxes, PQ, dt."""
+    faces = np.ascontiguousarray(faces, dtype=np.float64)
+    assert faces.shape[0] == 18
+    m = faces.shape[1]
+    out = np.empty((10, m))
+    lib().ora_flux_mhd_batch(int(method), gamma, krit, m, _p(faces), float(n[0]), float(n[1]),
+                             float(n[2]), rad, _p(out), nthreads)
     return out
 
 

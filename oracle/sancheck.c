@@ -46,6 +46,8 @@ static void faces(long n, double* in, int special) {
 
 static void check_fluxes(void) {
   const long n = 4096;
+  ora_phys ph0;
+  ora_phys_init(&ph0, 5.0 / 3.0, 0.05);
   double* in = malloc(sizeof(double) * 10 * n);
   double* out = malloc(sizeof(double) * 6 * n);
   int* it = malloc(sizeof(int) * 2 * n);
@@ -69,6 +71,41 @@ static void check_fluxes(void) {
       ora_flux_q_batch(qm[j], 5.0 / 3.0, 0.05, n, in, normals[k][0], normals[k][1], 0.5, out, 1);
   }
   ora_godunov_guess_batch(5.0 / 3.0, 0.05, n, in, 1, 0, 0.5, guess, out, it, ps);
+  /* MHD faces: the gas faces as rho/p/velocity, random fields, the normal
+   * field switched off, at the 0.001 switch and along the normal; every
+   * normal the frame rule knows and one it does not */
+  {
+    double* mh = malloc(sizeof(double) * 18 * n);
+    double* mo = malloc(sizeof(double) * 10 * n);
+    for (long i = 0; i < n; i++)
+      for (int s = 0; s < 2; s++) {
+        double* w = mh + 9 * s * n;
+        w[0 * n + i] = in[(4 * s + 0) * n + i];
+        w[1 * n + i] = 2 * urand() - 1;
+        w[2 * n + i] = in[(4 * s + 1) * n + i];
+        w[3 * n + i] = in[(4 * s + 2) * n + i];
+        w[4 * n + i] = in[(4 * s + 3) * n + i];
+        w[5 * n + i] = 2 * urand() - 1;
+        for (int c = 6; c < 9; c++) w[c * n + i] = (i % 5 == 0) ? 0.0 : 6 * urand() - 3;
+        if (i % 7 == 1) w[6 * n + i] = 0.0009 * 3.544907701811032;
+        if (i % 11 == 2) { w[7 * n + i] = 0.0; w[8 * n + i] = 0.0; }
+      }
+    static const double mn[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0},
+                                    {0, -1, 0}, {0, 0, -1}, {0.05, 0.05, 0.05}};
+    for (int k = 0; k < 7; k++) {
+      ora_flux_mhd_batch(AST_HLLC_MHD, 5.0 / 3.0, 0.05, n, mh, mn[k][0], mn[k][1], mn[k][2],
+                         0.5, mo, 1);
+      ora_flux_mhd_batch(AST_HLLD, 5.0 / 3.0, 0.05, n, mh, mn[k][0], mn[k][1], mn[k][2], 0.5,
+                         mo, 1);
+    }
+    ora_mhd_fan fan;
+    double Lm[9] = {1, 0.5, 1, 0.1, 0.2, 0.3, 2, 1, -1}, Rm[9] = {0.2, 0.1, 0.1, 0, 0, 0, 2, -1, 1};
+    double P[8], PQ;
+    ora_hlldq(&ph0, Lm, Rm, 1, 0, 0, 1.0, AST_HLLD, P, &PQ, &fan);
+    ora_hlldq(&ph0, Lm, Rm, 0, 0, -1, 1.0, AST_HLLC_MHD, P, &PQ, &fan);
+    free(mh);
+    free(mo);
+  }
   /* the reference's test kernel inputs and one face per exact pattern */
   ora_phys ph;
   ora_phys_init(&ph, 5.0 / 3.0, 0.05);

@@ -245,3 +245,130 @@ def pulsed_equivalent(g, TT):
     bc = (G.BC_DIRICHLET,) + tuple(g.bc[1:])
     rows_lo = g.y(np.arange(g.M)) < ylim
     return g.replace(bc=bc, bc_state=lo), g.replace(bc=bc, bc_state=st), rows_lo
+
+
+# ---- MHD faces (HLLDQ_Korolkov, Gas_Dinamic.cu:874) ---------------------------
+
+SPI4 = 3.544907701811032  # spi4, Header.h:69: the solver works with b = B / SPI4
+AXES = [(1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0),
+        (-1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, -1.0)]
+
+
+def mhd_faces(n, seed, axis=0, continuous_bn=True):
+    """(18, n) MHD face pairs (rho, Q, p, v1..v3, Bx..Bz left, then right):
+    densities and pressures over 3 and 3.5 decades, plasma beta from ~1e-3 to
+    ~1e3, Mach numbers up to ~5; with continuous_bn the field component along
+    `axis` is the same on both sides, as in a divergence-free field.  The
+    last rows hold the solver's special cases: a zero normal field (the
+    Alfven fan switched off), |b_n| just below and above 0.001 (the switch),
+    fields along the normal with beta < 1 (t = rho (S-u)(S-SM) - b_n^2 near
+    zero: the degenerate star states), and a field-free pair."""
+    rng = np.random.default_rng(seed)
+    F = np.empty((18, n))
+    for k in (0, 9):
+        F[k] = np.exp(rng.uniform(-3.0, 3.0, n))
+        F[k + 1] = rng.uniform(-2.0, 2.0, n) * F[k]
+        F[k + 2] = np.exp(rng.uniform(-4.0, 3.0, n))
+        c = np.sqrt(5.0 / 3.0 * F[k + 2] / F[k])
+        F[k + 3:k + 6] = rng.normal(0.0, 1.5, (3, n)) * c
+        F[k + 6:k + 9] = (rng.normal(0.0, 1.0, (3, n)) * np.sqrt(F[k + 2])
+                          * np.exp(rng.uniform(-1.5, 1.5, n)) * SPI4)
+    if continuous_bn:
+        F[15 + axis] = F[6 + axis]
+    m = max(n // 16, 1)
+    tail = n - 6 * m
+    if tail > 0:
+        rows = np.arange(tail, n)
+        seg = np.array_split(rows, 6)
+        for k in (0, 9):
+            F[k + 6 + axis, seg[0]] = 0.0                                   # b_n = 0
+            F[k + 6 + axis, seg[1]] = 0.0009 * SPI4 * np.sign(rng.normal(size=len(seg[1])))
+            F[k + 6 + axis, seg[2]] = 0.0011 * SPI4 * np.sign(rng.normal(size=len(seg[2])))
+            for j in range(3):                                              # field along n
+                if j != axis:
+                    F[k + 6 + j, seg[3]] = 0.0
+            F[k + 6 + axis, seg[3]] = np.sqrt(3.0 * F[k + 2, seg[3]]) * SPI4
+            F[k + 6:k + 9, seg[4]] = 0.0                                    # no field
+        for s in seg[1:4]:
+            F[15 + axis, s] = F[6 + axis, s]
+        # stationary contacts: one state, the right density tripled, u_n = 0
+        s = seg[5]
+        F[9:18, s] = F[0:9, s]
+        F[9, s] *= 3.0
+        F[3 + axis, s] = F[12 + axis, s] = 0.0
+    return F
+
+
+def mhd_state(W, nvec, gamma):
+    """Primitive pieces of (9, n) states along the normal nvec."""
+    rho, Q, p = W[0], W[1], W[2]
+    v = W[3:6]
+    b = W[6:9] / SPI4
+    nv = np.asarray(nvec, dtype=np.float64)[:, None]
+    un = (v * nv).sum(0)
+    bn = (b * nv).sum(0)
+    bb = (b * b).sum(0)
+    pT = p + 0.5 * bb
+    e = p / (gamma - 1.0) + 0.5 * rho * (v * v).sum(0) + 0.5 * bb
+    return rho, Q, p, v, b, un, bn, bb, pT, e
+
+
+def mhd_phys_flux(W, nvec, gamma):
+    """The ideal-MHD flux of states W (9, n) through the normal nvec in the
+    reference's output order (mass, momentum x/y/z, field x/y/z, energy),
+    and the scalar's flux, written in vector form from the equations."""
+    rho, Q, p, v, b, un, bn, bb, pT, e = mhd_state(W, nvec, gamma)
+    nv = np.asarray(nvec, dtype=np.float64)[:, None]
+    out = np.empty((8,) + rho.shape)
+    out[0] = rho * un
+    out[1:4] = rho * v * un + pT * nv - bn * b
+    out[4:7] = SPI4 * (un * b - bn * v)
+    out[7] = (e + pT) * un - bn * (v * b).sum(0)
+    return out, Q * un
+
+
+def mhd_fast_speed(W, nvec, gamma):
+    """Fast magnetosonic speed along nvec from its closed form."""
+    rho, Q, p, v, b, un, bn, bb, pT, e = mhd_state(W, nvec, gamma)
+    a2 = gamma * p / rho
+    s = a2 + bb / rho
+    return np.sqrt(0.5 * (s + np.sqrt(np.maximum(s * s - 4.0 * a2 * bn * bn / rho, 0.0))))
+
+
+def mhd_scale(F, gamma):
+    """Per-face natural scale of the MHD fluxes: the largest of each side's
+    (rho + rho|v|^2 + e + pT + |b|^2 + SPI4 |b| |v|) (|v| + c_f) and |Q| (|v| + c_f)."""
+    out = np.zeros(F.shape[1])
+    for k in (0, 9):
+        W = F[k:k + 9]
+        rho, Q, p, v, b, un, bn, bb, pT, e = mhd_state(W, (1.0, 0.0, 0.0), gamma)
+        vm = np.sqrt((v * v).sum(0))
+        cf = np.sqrt((gamma * p + bb) / rho)
+        sp = vm + cf
+        out = np.maximum(out, (rho + rho * vm * vm + e + pT + bb + SPI4 * np.sqrt(bb) * vm
+                               + np.abs(Q)) * sp)
+    return out
+
+
+def mhd_rotational(rng, left_going):
+    """An isolated rotational (Alfven) discontinuity with the normal along x:
+    rho, p, u_n, b_n and |b_t| continuous, b_t rotated by an angle, and
+    [v_t] = +-sign(b_n) [b_t] / sqrt(rho) (from rho (u - S)[v_t] = b_n [b_t]
+    at S = u -+ |b_n| / sqrt(rho)); u_n puts x/t = 0 between the
+    discontinuity and the contact, so the exact flux is that of the state
+    on the far side."""
+    rho, p = np.exp(rng.uniform(-1, 1)), np.exp(rng.uniform(-1, 1))
+    bn = rng.choice([-1.0, 1.0]) * rng.uniform(0.3, 1.5)
+    bt = rng.normal(0, 1, 2)
+    phi = rng.uniform(0.3, 3.0)
+    c, s = np.cos(phi), np.sin(phi)
+    btR = np.array([c * bt[0] - s * bt[1], s * bt[0] + c * bt[1]])
+    ca = abs(bn) / np.sqrt(rho)
+    sgn = np.sign(bn) * (1.0 if left_going else -1.0)
+    vt = rng.normal(0, 1, 2)
+    vtR = vt + sgn * (btR - bt) / np.sqrt(rho)
+    u_n = (0.5 if left_going else -0.5) * ca
+    Q = rng.uniform(-1, 1)
+    L = np.array([rho, Q, p, u_n, vt[0], vt[1], bn * SPI4, bt[0] * SPI4, bt[1] * SPI4])
+    R = np.array([rho, Q, p, u_n, vtR[0], vtR[1], bn * SPI4, btR[0] * SPI4, btR[1] * SPI4])
+    return L, R

@@ -48,6 +48,29 @@ def test_python_binding_covers_exports():
     assert not missing, missing
 
 
+def test_flux_mhd_argument_checks(lib):
+    """ast_flux_mhd_f64 refuses a bad method, gamma, a normal that picks no
+    frame, null and overlapping buffers, before any device work."""
+    import ctypes
+
+    from asteriy_2d_amd import grid as G
+    from asteriy_2d_amd import native
+    L = native.load()
+    buf = (ctypes.c_double * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    q = ctypes.c_void_p(p.value + 8)
+    far = ctypes.c_void_p(p.value + 4096)
+    f = L.ast_flux_mhd_f64
+    assert f(G.HLLC, 1, p, 1.0, 0.0, 0.0, 1.0, 5 / 3, 0.05, far, None) == 1
+    assert f(G.HLLD, 1, p, 1.0, 0.0, 0.0, 1.0, 1.0, 0.05, far, None) == 1
+    assert f(G.HLLD, 1, p, 0.05, -0.1, 0.02, 1.0, 5 / 3, 0.05, far, None) == 1
+    assert b"normal" in L.ast_last_error()
+    assert f(G.HLLD, 1, None, 1.0, 0.0, 0.0, 1.0, 5 / 3, 0.05, far, None) == 1
+    assert f(G.HLLD, 2, p, 1.0, 0.0, 0.0, 1.0, 5 / 3, 0.05, q, None) == 1
+    assert b"overlap" in L.ast_last_error()
+    assert f(G.HLLC_MHD, 0, None, -1.0, 0.0, 0.0, 1.0, 5 / 3, 0.05, None, None) == 0
+
+
 def test_no_device_needed_for_validation(lib):
     from asteriy_2d_amd import grid as G
     from asteriy_2d_amd import native
