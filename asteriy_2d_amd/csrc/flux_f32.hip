@@ -78,6 +78,12 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
 // lanes of a wave mostly hold faces of one pattern.  Results depend only on
 // each face's own data: the order changes no bit.
 //
+// The tile's faces are kept in LDS from the classifying (coalesced) loads on:
+// the solve reads its permuted faces there and leaves its fluxes there, and
+// the tile's fluxes go out with coalesced stores (measured: re-reading the
+// faces from global memory in sorted order, 32 scattered loads per lane and
+// tile, took 60% of the kernel's time).
+//
 // With the learned newton start, a first kernel (kGuessPass) evaluates the
 // network for the strong two-shock faces only -- they are contiguous after
 // the sort, so its waves are full of faces that use the guess -- and leaves
@@ -86,16 +92,15 @@ __global__ __launch_bounds__(FNT) void flux_kernel(size_t n, const float* __rest
 // thus carries no network registers, and the network's own kernel runs only
 // on the faces it serves.
 constexpr int SORT_NT = 256;
-constexpr int SORT_PER = 4;                  // faces per thread per tile
+constexpr int SORT_PER = 2;                  // faces per thread per tile
 constexpr int STILE = SORT_NT * SORT_PER;    // faces per tile
 constexpr int kSolve = 0, kGuessPass = 1, kSolveGuessed = 2;
 
-AST_DEV void load_face32(const float* __restrict__ in, size_t n, size_t i, float n1, float n2,
-                         PrimF& L, PrimF& R) {
-  const float vxl = in[2 * n + i], vyl = in[3 * n + i];
-  const float vxr = in[6 * n + i], vyr = in[7 * n + i];
-  L = PrimF{in[i], in[n + i], vxl * n1 + vyl * n2, -vxl * n2 + vyl * n1};
-  R = PrimF{in[4 * n + i], in[5 * n + i], vxr * n1 + vyr * n2, -vxr * n2 + vyr * n1};
+// The face's eight words (rhoL, pL, vxL, vyL, rhoR, pR, vxR, vyR) in the
+// frame of the normal (n1, n2).
+AST_DEV void face_frame(const float* w, float n1, float n2, PrimF& L, PrimF& R) {
+  L = PrimF{w[0], w[1], w[2] * n1 + w[3] * n2, -w[2] * n2 + w[3] * n1};
+  R = PrimF{w[4], w[5], w[6] * n1 + w[7] * n2, -w[6] * n2 + w[7] * n1};
 }
 
 template <bool STATS, int MODE>
@@ -109,6 +114,7 @@ __global__ __launch_bounds__(SORT_NT) void exact_sorted_kernel(size_t n,
   __shared__ uint32_t redi[(SORT_NT / 64) * 10];
   __shared__ unsigned short perm[STILE];
   __shared__ int cnt[kPatternKeys + 1], cur[kPatternKeys + 1];
+  __shared__ float stage[8][STILE];  // the tile's faces, then (5 planes) its fluxes
   typename std::conditional<STATS, StatsF32, NoStats>::type st;
   st.clear();
   const size_t ntiles = (n + STILE - 1) / STILE;
@@ -117,15 +123,21 @@ __global__ __launch_bounds__(SORT_NT) void exact_sorted_kernel(size_t n,
     const int nv = (int)min((size_t)STILE, n - base);
     if (threadIdx.x <= kPatternKeys) cnt[threadIdx.x] = 0;
     __syncthreads();
-    // classify (coalesced loads) and count
+    // classify (coalesced loads, kept in LDS) and count
     int key[SORT_PER];
 #pragma unroll
     for (int j = 0; j < SORT_PER; j++) {
       const int li = j * SORT_NT + threadIdx.x;
       key[j] = kPatternKeys;  // past the end of the batch
       if (li < nv) {
+        float w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          w[k] = in[k * n + base + li];
+          stage[k][li] = w[k];
+        }
         PrimF L, R;
-        load_face32(in, n, base + li, n1, n2, L, R);
+        face_frame(w, n1, n2, L, R);
         key[j] = pattern_key(L, R, ex);
         if (!STRONG && key[j] == 4) key[j] = 3;  // no network: strong faces are plain ones
       }
@@ -155,8 +167,11 @@ __global__ __launch_bounds__(SORT_NT) void exact_sorted_kernel(size_t n,
       const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
       for (int w0 = s0 + 64 * wave; w0 < s1; w0 += SORT_NT) {  // wave-uniform
         const int li = perm[min(w0 + lane, s1 - 1)];
+        float w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) w[k] = stage[k][li];
         PrimF L, R;
-        load_face32(in, n, base + li, n1, n2, L, R);
+        face_frame(w, n1, n2, L, R);
         const bool sw = L.p > R.p;
         float fe[mlp::IN];
         const float lp = mlp::features<float>(sw ? R.r : L.r, sw ? R.p : L.p, sw ? -R.u : L.u,
@@ -174,16 +189,21 @@ __global__ __launch_bounds__(SORT_NT) void exact_sorted_kernel(size_t n,
         if (pos < nfaces) {
           const int li = perm[pos];
           const size_t i = base + li;
+          float w[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) w[k] = stage[k][li];
           PrimF L, R;
-          load_face32(in, n, i, n1, n2, L, R);
+          face_frame(w, n1, n2, L, R);
           const float g = (MODE == kSolveGuessed && pos >= s0 && pos < s1) ? out[4 * n + i] : -1.f;
           float UU;
           const FluxF f = exact_f32<false>(L, R, ex, UU, fi, g);
-          out[i] = f.m;
-          out[n + i] = f.e;
-          out[2 * n + i] = f.un * n1 - f.ut * n2;
-          out[3 * n + i] = f.un * n2 + f.ut * n1;
-          out[4 * n + i] = krad * frcp(UU);
+          // this lane alone holds face li: its words are read, its fluxes
+          // take their place
+          stage[0][li] = f.m;
+          stage[1][li] = f.e;
+          stage[2][li] = f.un * n1 - f.ut * n2;
+          stage[3][li] = f.un * n2 + f.ut * n1;
+          stage[4][li] = krad * frcp(UU);
           if (info) {
             info[2 * i] = fi.it;
             info[2 * i + 1] = (fi.pat == kNoPattern ? -1 : fi.pat + 1) | (fi.capped ? 8 : 0) |
@@ -192,8 +212,18 @@ __global__ __launch_bounds__(SORT_NT) void exact_sorted_kernel(size_t n,
         }
         st.add(fi);
       }
+      __syncthreads();
+      // the tile's fluxes, coalesced
+#pragma unroll
+      for (int j = 0; j < SORT_PER; j++) {
+        const int li = j * SORT_NT + threadIdx.x;
+        if (li < nv) {
+#pragma unroll
+          for (int k = 0; k < 5; k++) out[k * n + base + li] = stage[k][li];
+        }
+      }
     }
-    __syncthreads();  // perm and the counters are reused by the next tile
+    __syncthreads();  // stage, perm and the counters are reused by the next tile
   }
   if constexpr (STATS) block_stats_f32_to<SORT_NT>(st, redi, stats);
 }
@@ -207,8 +237,11 @@ template <int METHOD>
 static int launch(size_t n, const float* in, float n1, float n2, float krad, const ExactCtx& ex,
                   float* out, ast_stats_t* stats, int* info, hipStream_t stream) {
   if constexpr (METHOD == AST_EXACT) {
+    // one tile per workgroup (measured on 2^24 faces: 512-face tiles, one per
+    // workgroup, 346 us; 1024-face tiles 367-403 us; 4096 workgroups looping
+    // over the tiles 389-427 us), a loop over tiles beyond 2^20 workgroups
     size_t blocks = (n + STILE - 1) / STILE;
-    if (blocks > 256 * 16) blocks = 256 * 16;
+    if (blocks > (size_t(1) << 20)) blocks = size_t(1) << 20;
     const dim3 gr((unsigned)blocks), bl(SORT_NT);
     const bool st = stats != nullptr;
     if (ex.mlp != nullptr) {
