@@ -682,8 +682,8 @@ def main(argv=None):
             fl_nudge.append(_l1(_fp64_fp32_state(x[0], x[1], x[2], Kb, nudge_seed=k)[0][..., 0],
                                 ref[0][..., 0]))
         bt.close()
-        # a chaotic instance amplifies the same storage error into the two
-        # realisations unevenly: the larger of the two is its floor
+        # the storage error of an instance varies with its rounding realisation
+        # (up to ~2x between these two): the larger of the two is its floor
         floor = np.maximum(np.array(fl_store), np.array(fl_nudge))
         ratio = np.array(errs) / np.maximum(floor, 1e-300)
         ratio_nearest = np.array(errs) / np.maximum(np.array(fl_store), 1e-300)
