@@ -79,6 +79,38 @@ def test_flux_f32_vs_f64(dev, method):
         assert d["exact_faces"] == F.shape[1]
 
 
+@pytest.mark.parametrize("learned", [False, True])
+def test_exact_f32_sorted_tiles_are_order_free(dev, learned):
+    """The FP32 exact flux sorts each tile's faces by wave pattern in LDS and
+    solves them in that order: a face's flux depends only on its own words,
+    so every face of a batch -- whole tiles, the partial last tile, a batch
+    shorter than one tile -- gets the same bits whatever batch holds it, and
+    so does its dt (with and without the learned start's guess pass)."""
+    import ctypes
+
+    import torch
+    from asteriy_2d_amd import learned as LG
+    from asteriy_2d_amd import native
+    net = LG.LearnedGuess.load() if learned else None  # held: its buffer is read on the GPU
+    mlp = ctypes.c_void_p(net.buffer.data_ptr()) if learned else None
+    F = _t(facegen.faces(5000, seed=23), dev).float()
+
+    def run(Fs):
+        Fs = Fs.contiguous()
+        out = torch.empty((5, Fs.shape[1]), dtype=torch.float32, device=dev)
+        native.call("ast_flux_f32_info", G.EXACT, Fs.shape[1], ctypes.c_void_p(Fs.data_ptr()),
+                    1.0, 0.0, 1.0, 5.0 / 3.0, 0.05, mlp, ctypes.c_void_p(out.data_ptr()), None,
+                    None, None)
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+    full = run(F)
+    for lo, hi in ((0, 1000), (777, 5000), (4990, 5000), (512, 1024)):
+        part = run(F[:, lo:hi])
+        assert np.array_equal(part, full[:, lo:hi], equal_nan=True), (lo, hi)
+    rev = run(torch.flip(F, dims=[1]))
+    assert np.array_equal(rev[:, ::-1], full, equal_nan=True)
+
+
 def test_exact_f32_fp64_islands(dev):
     """Faces whose p* equation cancels in FP32 -- two rarefactions near vacuum
     and shock + rarefaction with pI < 1e-4 pII -- form that difference in FP64
