@@ -37,6 +37,7 @@ SOURCES = [
     ("misc.hip", PARITY),
     ("flux_f64.hip", PARITY),
     ("flux_mhd_f64.hip", PARITY),
+    ("mc.hip", PARITY),
     ("step_f64.hip", PARITY),
     ("flux_f32.hip", FAST),
     ("step_f32.hip", FAST),

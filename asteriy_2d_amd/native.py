@@ -39,6 +39,9 @@ SIGNATURES = {
                                 _c_double, _c_double, _vp, _vp]),
     "ast_flux_mhd_f64": (_c_int, [_c_int, _c_size, _vp, _c_double, _c_double, _c_double,
                                   _c_double, _c_double, _c_double, _vp, _vp]),
+    "ast_mc_rates_f64": (_c_int, [_c_size, _vp, _vp, _vp, _vp]),
+    "ast_mc_cell_exit_f64": (_c_int, [_gp, _c_size, _vp, _vp, _vp, _vp, _vp]),
+    "ast_mc_belong_point_f64": (_c_int, [_gp, _c_size, _vp, _vp, _vp]),
     "ast_step_f64": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp]),
     "ast_step_f64_src": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
                                   ctypes.POINTER(CSources), _vp, _vp]),

@@ -224,6 +224,49 @@ def HLLDQ_Korolkov(faces, n1, n2, n3, rad, metod, **kw):
     return flux_mhd_batch(M_HLLD if metod == 3 else M_HLLC_MHD, faces, n1, n2, n3, rad, **kw)
 
 
+def mc_rates(u, cp, out=None, stream=None):
+    """Velosity_1/2/3 (Gas_Dinamic.cu:1566-1602) of float64 device tensors
+    u, cp (n each): returns (3, n)."""
+    import torch
+    if u.shape != cp.shape or u.dim() != 1 or u.dtype != torch.float64 or cp.dtype != torch.float64:
+        raise ValueError("u and cp must be float64 vectors of one length")
+    u, cp = u.contiguous(), cp.contiguous()
+    n = u.shape[0]
+    if out is None:
+        out = torch.empty((3, n), dtype=torch.float64, device=u.device)
+    native.call("ast_mc_rates_f64", n, _ptr(u), _ptr(cp), _ptr(out), _stream(stream))
+    return out
+
+
+def mc_cell_exit(grid, particles, stream=None):
+    """Peresechenie (Gas_Dinamic.cu:1492) of the particles, a (8, n) float64
+    device tensor (x0, y0, x, y, z, Vx, Vy, Vz): returns (t, mode, hit)."""
+    import torch
+    if particles.dim() != 2 or particles.shape[0] != 8 or particles.dtype != torch.float64:
+        raise ValueError("particles must be an (8, n) float64 tensor")
+    p = particles.contiguous()
+    n = p.shape[1]
+    t = torch.empty(n, dtype=torch.float64, device=p.device)
+    mode = torch.empty(n, dtype=torch.int32, device=p.device)
+    hit = torch.empty(n, dtype=torch.int32, device=p.device)
+    native.call("ast_mc_cell_exit_f64", native.grid_ref(grid), n, _ptr(p), _ptr(t), _ptr(mode),
+                _ptr(hit), _stream(stream))
+    return t, mode, hit
+
+
+def mc_belong_point(grid, xy, stream=None):
+    """Belong_point (Gas_Dinamic.cu:1559) of the points xy, a (2, n) float64
+    device tensor (x, r): returns the (2, n) int32 cell indices (n, m)."""
+    import torch
+    if xy.dim() != 2 or xy.shape[0] != 2 or xy.dtype != torch.float64:
+        raise ValueError("xy must be a (2, n) float64 tensor")
+    xy = xy.contiguous()
+    nm = torch.empty((2, xy.shape[1]), dtype=torch.int32, device=xy.device)
+    native.call("ast_mc_belong_point_f64", native.grid_ref(grid), xy.shape[1], _ptr(xy), _ptr(nm),
+                _stream(stream))
+    return nm
+
+
 def conservation_sums(grid, s, u, stream=None):
     """Compensated FP64 (mass, energy) integrals of the state (device)."""
     import torch

@@ -182,6 +182,27 @@ int ast_flux_mhd_f64(int method, size_t n, const double* in, double n1, double n
                      double n3, double rad, double gamma, double krit, double* out,
                      void* stream);
 
+/* ---- host helpers of the Monte-Carlo neutral-hydrogen code --------------
+ * The particle-tracing and rate functions of Gas_Dinamic.cu:1468-1602
+ * (declared Header.h:135-146) over device arrays, one item per lane; the
+ * per-particle __host__ __device__ forms are in asteriy_mc.hpp.  Results are
+ * the reference's, its thresholds and its nine-digit sqrt(pi) (Header.h:65)
+ * included.
+ * ast_mc_rates_f64: Velosity_1, Velosity_2, Velosity_3 (Gas_Dinamic.cu:1566,
+ *   1578, 1591) of n pairs (u[i], cp[i]); out = 3 planes of n.
+ * ast_mc_cell_exit_f64: Peresechenie (Gas_Dinamic.cu:1492) of n particles
+ *   with the grid's dx, dy; in = 8 planes (x0, y0, x, y, z, Vx, Vy, Vz: the
+ *   cell centre, the position, the velocity); out: t[i], mode[i] (1..4: the
+ *   side, 0 none) and hit[i] (the function's value, 0/1).
+ * ast_mc_belong_point_f64: Belong_point (Gas_Dinamic.cu:1559) of n points;
+ *   xy = 2 planes (x, r); nm = 2 int planes (n, m), the grid's cell indices
+ *   (the reference's (int)(y/dy) with the grid's own first cell centre). */
+int ast_mc_rates_f64(size_t n, const double* u, const double* cp, double* out, void* stream);
+int ast_mc_cell_exit_f64(const ast_grid_t* g, size_t n, const double* in, double* t, int* mode,
+                         int* hit, void* stream);
+int ast_mc_belong_point_f64(const ast_grid_t* g, size_t n, const double* xy, int* nm,
+                            void* stream);
+
 /* ---- the step ------------------------------------------------------------
  * Replaces
  *   __global__ void Kernel_TVD(double2* s, double2* u, double2* s2,

@@ -38,6 +38,14 @@
  *                                 B = 0, the integral consistency of the fan
  *                                 with the HLL average, the fast speed's
  *                                 closed forms in the returned dt.
+ *  Monte-Carlo host helpers      : Velosity_1/2/3 against quadratures of
+ *  (Gas_Dinamic.cu:1468-1602)      the Maxwellian averages they close
+ *                                 (<|w|>, the momentum-exchange integral,
+ *                                 <|w|^3>/<|w|^2>) and their series below
+ *                                 u = 0.001; Peresechenie against a brute
+ *                                 march of the ray through the cell;
+ *                                 peresich by substitution; Belong_point
+ *                                 and minplus by cases.
  *  ora_godunov (+newton/devtwo) : independent Toro exact solver (toro.c),
  *                                 Toro's published test table, the
  *                                 reference's own `test` kernel KAT
@@ -857,6 +865,103 @@ void ora_flux_mhd_batch(int method, double gamma, double krit, long n,
     out[8 * n + i] = PQ;
     out[9 * n + i] = t;
   }
+}
+
+/* ---- host helpers of the Monte-Carlo neutral-hydrogen code ------------ */
+
+/* Header.h:64-65 (pi exact to double; sqrtpi truncated to 9 digits, as the
+ * reference writes it). */
+#define O_PI 3.14159265358979323846
+#define O_SQRTPI 1.77245385
+
+/* minplus, Gas_Dinamic.cu:1468-1490 */
+double ora_minplus(double x, double y) {
+  if (x < 0.00000000001 && y < 0.00000000001) return -1.0;
+  else if (x < 0.00000000001) return y;
+  else if (y < 0.00000000001) return x;
+  else if (x < y) return x;
+  else return y;
+}
+
+/* peresich, Gas_Dinamic.cu:1543-1557 */
+void ora_peresich(double y, double z, double Vy, double Vz, double R, double* t1, double* t2) {
+  const double b = (2.0 * y * Vy + 2.0 * z * Vz);
+  const double a = (Vy * Vy + Vz * Vz);
+  double D = b * b - 4.0 * (y * y + z * z - R * R) * a;
+  if (D < 0) {
+    *t1 = -1.0;
+    *t2 = -1.0;
+    return;
+  }
+  D = sqrt(D);
+  *t1 = (-b + D) / (2.0 * a);
+  *t2 = (-b - D) / (2.0 * a);
+}
+
+/* Peresechenie, Gas_Dinamic.cu:1492-1541, with the cell sizes dx, dy of the
+ * grid in place of the Header.h macros */
+int ora_peresechenie(const ast_grid_t* g, double x0, double y0, double x, double y, double z,
+                     double Vx, double Vy, double Vz, int* mode, double* t) {
+  const double dx = g->dx, dy = g->dy;
+  *mode = 0;
+  double t1 = -1.0, t2 = -1.0, t3 = -1.0, t4 = -1.0, t5 = -1.0, t6 = -1.0;
+  if (fabs(Vx) > 0.0000001) {
+    t1 = (x0 + dx / 2.0 - x) / Vx;
+    t2 = (x0 - dx / 2.0 - x) / Vx;
+  }
+  if (Vy * Vy + Vz * Vz > 0.0000001) {
+    ora_peresich(y, z, Vy, Vz, y0 + dy / 2.0, &t3, &t4);
+    ora_peresich(y, z, Vy, Vz, y0 - dy / 2.0, &t5, &t6);
+    t3 = ora_minplus(t3, t4);
+    t4 = ora_minplus(t5, t6);
+  }
+  *t = 100000000.0;
+  if (*t > t1 && t1 > 0.000000001) { *t = t1; *mode = 1; }
+  if (*t > t2 && t2 > 0.000000001) { *t = t2; *mode = 2; }
+  if (*t > t3 && t3 > 0.000000001) { *t = t3; *mode = 3; }
+  if (*t > t4 && t4 > 0.000000001) { *t = t4; *mode = 4; }
+  if (fabs(Vx) > 0.0000001 && t1 < 0.000000001 && t2 < 0.000000001) return 0;
+  if (*mode == 0) return 0;
+  return 1;
+}
+
+/* Belong_point, Gas_Dinamic.cu:1559-1564 (the reference's y/dy assumes
+ * cell centres at (m + 1/2) dy, y_min = dy/2) */
+void ora_belong_point(const ast_grid_t* g, double x, double y, int* n, int* m) {
+  *m = (int)(y / g->dy);
+  *n = (int)((x - g->x_min + g->dx / 2.0) / g->dx);
+}
+
+/* Velosity_1, Gas_Dinamic.cu:1566-1576 */
+double ora_velosity_1(double u, double cp) {
+  if (u < 0.001)
+    return 2.0 * cp / O_SQRTPI + 2.0 * u * u / (3.0 * cp * O_SQRTPI) -
+           u * u * u * u / (15.0 * cp * cp * cp * O_SQRTPI);
+  return exp(-u * u / (cp * cp)) * cp / O_SQRTPI + (u + (cp * cp) / (2.0 * u)) * erf(u / cp);
+}
+
+/* Velosity_2, Gas_Dinamic.cu:1578-1589 */
+double ora_velosity_2(double u, double cp) {
+  if (u < 0.001)
+    return (8.0 / 3.0) * (cp * cp) * (cp * cp) * O_PI * u +
+           (8.0 / 15.0) * (cp * cp) * O_PI * u * u * u -
+           (4.0 / 105.0) * O_PI * (u * u) * (u * u) * u;
+  return cp * cp * cp * O_PI *
+         (exp(-u * u / (cp * cp)) * cp * u * 2.0 * ((cp * cp) + 2.0 * (u * u)) +
+          O_SQRTPI * (4.0 * (u * u) * (u * u) + 4.0 * cp * cp * (u * u) - (cp * cp) * (cp * cp)) *
+              erf(u / cp)) /
+         (4.0 * u * u);
+}
+
+/* Velosity_3, Gas_Dinamic.cu:1591-1602 */
+double ora_velosity_3(double u, double cp) {
+  if (u < 0.001)
+    return 8.0 * cp / (3.0 * O_SQRTPI) + 8.0 * u * u / (9.0 * cp * O_SQRTPI) -
+           44.0 * u * u * u * u / (135.0 * cp * cp * cp * O_SQRTPI);
+  return exp(-u * u / (cp * cp)) * cp * (5.0 * (cp * cp) + 2.0 * (u * u)) /
+             (O_SQRTPI * (3.0 * (cp * cp) + 2.0 * (u * u))) +
+         (4.0 * (u * u) * (u * u) + 12.0 * cp * cp * (u * u) + 3.0 * (cp * cp) * (cp * cp)) *
+             erf(u / cp) / (2.0 * u * (3.0 * (cp * cp) + 2.0 * (u * u)));
 }
 
 /* lev, kernel.cu:751-765 */

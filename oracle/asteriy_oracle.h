@@ -68,6 +68,15 @@ double ora_hlldq(const ora_phys*, const double* L, const double* R, double n1,
 void ora_flux_mhd_batch(int method, double gamma, double krit, long n,
                         const double* in, double n1, double n2, double n3,
                         double rad, double* out, int nthreads);
+/* host helpers of the Monte-Carlo code (Gas_Dinamic.cu:1468-1602) */
+double ora_minplus(double x, double y);
+void ora_peresich(double y, double z, double Vy, double Vz, double R, double* t1, double* t2);
+int ora_peresechenie(const ast_grid_t* g, double x0, double y0, double x, double y, double z,
+                     double Vx, double Vy, double Vz, int* mode, double* t);
+void ora_belong_point(const ast_grid_t* g, double x, double y, int* n, int* m);
+double ora_velosity_1(double u, double cp);
+double ora_velosity_2(double u, double cp);
+double ora_velosity_3(double u, double cp);
 double ora_godunov(const ora_phys*, const double* Ls, const double* Lu,
                    const double* Rs, const double* Ru, double n1, double n2,
                    double* Ps, double* Pu, double rad);

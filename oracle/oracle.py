@@ -114,6 +114,16 @@ def lib():
         L.ora_flux_mhd_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_long, _dp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
+        L.ora_minplus.restype = ctypes.c_double
+        L.ora_minplus.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.ora_peresich.argtypes = [ctypes.c_double] * 5 + [_dp, _dp]
+        L.ora_peresechenie.restype = ctypes.c_int
+        L.ora_peresechenie.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 8 + [_ip, _dp]
+        L.ora_belong_point.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _ip,
+                                       _ip]
+        for fn in ("ora_velosity_1", "ora_velosity_2", "ora_velosity_3"):
+            getattr(L, fn).restype = ctypes.c_double
+            getattr(L, fn).argtypes = [ctypes.c_double, ctypes.c_double]
         L.ora_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
                                ctypes.c_int, ctypes.c_int]
         L.ora_step_src.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -229,6 +239,40 @@ def flux_mhd_batch(method: int, faces: np.ndarray, n=(1.0, 0.0, 0.0), rad=1.0,
     lib().ora_flux_mhd_batch(int(method), gamma, krit, m, _p(faces), float(n[0]), float(n[1]),
                              float(n[2]), rad, _p(out), nthreads)
     return out
+
+
+def minplus(x, y):
+    return lib().ora_minplus(x, y)
+
+
+def peresich(y, z, Vy, Vz, R):
+    """Gas_Dinamic.cu:1543: the times at which the ray (y, z) + t (Vy, Vz)
+    meets the circle of radius R, or (-1, -1)."""
+    t1, t2 = ctypes.c_double(), ctypes.c_double()
+    lib().ora_peresich(y, z, Vy, Vz, R, ctypes.byref(t1), ctypes.byref(t2))
+    return t1.value, t2.value
+
+
+def peresechenie(grid, x0, y0, x, y, z, Vx, Vy, Vz):
+    """Gas_Dinamic.cu:1492: (hit, mode, t) of the ray leaving the cell with
+    centre (x0, y0) of the grid's dx, dy."""
+    cg = grid.to_c()
+    mode, t = ctypes.c_int(), ctypes.c_double()
+    hit = lib().ora_peresechenie(ctypes.byref(cg), x0, y0, x, y, z, Vx, Vy, Vz,
+                                 ctypes.byref(mode), ctypes.byref(t))
+    return bool(hit), mode.value, t.value
+
+
+def belong_point(grid, x, y):
+    cg = grid.to_c()
+    n, m = ctypes.c_int(), ctypes.c_int()
+    lib().ora_belong_point(ctypes.byref(cg), x, y, ctypes.byref(n), ctypes.byref(m))
+    return n.value, m.value
+
+
+def velosity(k: int, u: float, cp: float) -> float:
+    """Velosity_k (Gas_Dinamic.cu:1566, 1578, 1591)."""
+    return getattr(lib(), f"ora_velosity_{k}")(u, cp)
 
 
 def godunov_info(Ls, Lu, Rs, Ru, n1=1.0, n2=0.0, rad=1.0, gamma=5.0 / 3.0, krit=0.05,

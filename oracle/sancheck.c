@@ -106,6 +106,32 @@ static void check_fluxes(void) {
     free(mh);
     free(mo);
   }
+  /* the Monte-Carlo helpers: rates across the series switch and at u = 0,
+   * cell exits from random, axial, radial and resting particles, points */
+  {
+    ast_grid_t gm;
+    memset(&gm, 0, sizeof gm);
+    gm.N = 16;
+    gm.M = 8;
+    gm.x_min = -1.0;
+    gm.dx = 0.125;
+    gm.dy = 0.125;
+    gm.y_min = 0.0625;
+    double acc = 0.0;
+    for (long i = 0; i < n; i++) {
+      const double cp = uexp(-5, 5), u = (i % 3 == 0) ? 0.001 * urand() : cp * uexp(-8, 3);
+      acc += ora_velosity_1(u, cp) + ora_velosity_2(u, cp) + ora_velosity_3(u, cp);
+      int mode, cn, cm;
+      double t;
+      const double x0 = gm.x_min + (i % 16) * gm.dx, y0 = gm.y_min + (i % 8) * gm.dy;
+      const double vx = (i % 5 == 0) ? 0.0 : 2 * urand() - 1, vy = (i % 7 == 0) ? 0.0 : 2 * urand() - 1;
+      acc += ora_peresechenie(&gm, x0, y0, x0 + 0.01 * (urand() - 0.5), y0, 0.0, vx, vy,
+                              (i % 11 == 0) ? 0.0 : 2 * urand() - 1, &mode, &t);
+      ora_belong_point(&gm, x0 + 0.05 * urand(), y0, &cn, &cm);
+      acc += t + mode + cn + cm + ora_minplus(2 * urand() - 1, 2 * urand() - 1);
+    }
+    if (acc != acc) printf("(nan in the helper sums)\n");
+  }
   /* the reference's test kernel inputs and one face per exact pattern */
   ora_phys ph;
   ora_phys_init(&ph, 5.0 / 3.0, 0.05);

@@ -372,3 +372,39 @@ def mhd_rotational(rng, left_going):
     L = np.array([rho, Q, p, u_n, vt[0], vt[1], bn * SPI4, bt[0] * SPI4, bt[1] * SPI4])
     R = np.array([rho, Q, p, u_n, vtR[0], vtR[1], bn * SPI4, btR[0] * SPI4, btR[1] * SPI4])
     return L, R
+
+
+# ---- the Monte-Carlo helpers (Gas_Dinamic.cu:1468-1602) -----------------------
+
+def mc_rate_pairs(n, seed):
+    """(2, n) pairs (u, cp): speeds over five decades, a share below the
+    u = 0.001 series switch and at it, thermal speeds over three decades."""
+    rng = np.random.default_rng(seed)
+    cp = np.exp(rng.uniform(-3.0, 3.0, n))
+    u = cp * np.exp(rng.uniform(-6.0, 2.5, n))
+    k = n // 8
+    u[:k] = rng.uniform(0.0, 0.001, k)
+    u[k:2 * k] = 0.001 * (1.0 + rng.uniform(-1e-9, 1e-9, k))
+    u[2 * k] = 0.0
+    return np.ascontiguousarray(np.stack([u, cp]))
+
+
+def mc_particles(grid, n, seed):
+    """(8, n) particles (x0, y0, x, y, z, Vx, Vy, Vz) inside random cells of
+    the grid (the axis cells included), with random velocities, some purely
+    axial, some purely radial, some at rest."""
+    rng = np.random.default_rng(seed)
+    ci = rng.integers(0, grid.N, n)
+    cj = rng.integers(0, grid.M, n)
+    x0 = grid.x_min + ci * grid.dx
+    y0 = grid.y_min + cj * grid.dy
+    x = x0 + rng.uniform(-0.45, 0.45, n) * grid.dx
+    r = y0 + rng.uniform(-0.45, 0.45, n) * grid.dy
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    V = rng.normal(0.0, 1.0, (3, n))
+    k = n // 10
+    V[1:, :k] = 0.0
+    V[0, k:2 * k] = 0.0
+    V[:, 2 * k:2 * k + 3] = 0.0
+    return np.ascontiguousarray(np.stack([x0, y0, x, r * np.cos(phi), r * np.sin(phi),
+                                          V[0], V[1], V[2]]))
