@@ -95,6 +95,23 @@ def wind(N: int = 2048, M: int = 1024, method: int = HLLC, steps: int = 5000,
     return Problem(f"wind{N}x{M}", g, s, u, method, steps)
 
 
+def korolkov_initial_state(g: Grid):
+    """InitialConditions (KorolkovSolver.cu:35-53): inside the radius 100 of
+    the origin (x^2 + y^2 < 10000) rho = 8, p = 2, u = (-90, 0); elsewhere
+    rho = 1, p = 1, u = (-1, 0).  The reference defines it beside its own
+    Initialization (KorolkovSolver.cu:56) and calls neither from the live
+    main; the state is given for runs of that setup."""
+    x = g.x(np.arange(g.N))[None, :]
+    y = g.y(np.arange(g.M))[:, None]
+    inside = np.broadcast_to(x * x + y * y < 10000, (g.M, g.N))
+    s = np.empty((g.M, g.N, 2))
+    u = np.zeros((g.M, g.N, 2))
+    s[..., 0] = np.where(inside, 8.0, 1.0)
+    s[..., 1] = np.where(inside, 2.0, 1.0)
+    u[..., 0] = np.where(inside, -90.0, -1.0)
+    return s, u
+
+
 def random_state(g: Grid, seed: int = 0, smooth: bool = False):
     """A random but physical state (positive rho, p) for parity tests."""
     rng = np.random.default_rng(seed)

@@ -666,3 +666,17 @@ def test_pulsed_inflow_row_by_row(ora, TT):
     assert pT == min(lT, hT)
     moved = not np.array_equal(ls, hs)
     assert moved == (TT < g.bc_pulse[3] and math.sin(TT * g.bc_pulse[1]) != 0.0)
+
+
+def test_korolkov_initial_state():
+    """InitialConditions (KorolkovSolver.cu:35-53) on a grid straddling its
+    radius-100 disc: the two states and where they meet."""
+    g = G.Grid.from_extent(64, 32, -150.0, 150.0, 150.0)
+    s, u = problems.korolkov_initial_state(g)
+    x = g.x(np.arange(g.N))[None, :]
+    y = g.y(np.arange(g.M))[:, None]
+    R = np.sqrt(x * x + y * y)
+    ins, out = R < 100.0 - 1e-9, R > 100.0 + 1e-9
+    assert ins.any() and out.any()
+    assert np.all(s[ins] == (8.0, 2.0)) and np.all(u[ins] == (-90.0, 0.0))
+    assert np.all(s[out] == (1.0, 1.0)) and np.all(u[out] == (-1.0, 0.0))
