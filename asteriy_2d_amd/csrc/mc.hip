@@ -23,10 +23,11 @@ __global__ __launch_bounds__(MC_NT) void rates_kernel(size_t n, const double* __
                                                       double* __restrict__ out) {
   const size_t stride = (size_t)gridDim.x * MC_NT;
   for (size_t i = (size_t)blockIdx.x * MC_NT + threadIdx.x; i < n; i += stride) {
-    const double a = u[i], c = cp[i];
-    out[i] = asteriy::mc::velosity_1(a, c);
-    out[n + i] = asteriy::mc::velosity_2(a, c);
-    out[2 * n + i] = asteriy::mc::velosity_3(a, c);
+    double v1, v2, v3;
+    asteriy::mc::velosities(u[i], cp[i], v1, v2, v3);
+    out[i] = v1;
+    out[n + i] = v2;
+    out[2 * n + i] = v3;
   }
 }
 

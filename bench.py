@@ -172,6 +172,7 @@ def _flux_microbenchmark(mdl, n: int = 1 << 24, reps: int = 5):
                      "f64_hbm_frac": round(128.0 * n / (tq * 1e-3) / 8e12, 4)}
     del Fq, oq
     res.update(_mhd_microbenchmark(n, reps))
+    res["mc_helpers"] = _mc_microbenchmark(n, reps)
     return res
 
 
@@ -223,6 +224,67 @@ def _mhd_microbenchmark(n: int = 1 << 24, reps: int = 5):
         res[name] = {"f64_gfaces": round(n / tm / 1e6, 3),
                      "f64_hbm_frac": round(224.0 * n / (tm * 1e-3) / 8e12, 4)}
     del Fm, om
+    return res
+
+
+def _mc_microbenchmark(n: int = 1 << 24, reps: int = 5):
+    """The Monte-Carlo helpers over device arrays: the three rates of n seeded
+    (u, cp) pairs (16 B in, 24 B out per pair) and the cell exits of n
+    particles in the wind grid's cells (64 B in, 16 B out per particle):
+    items/s over `reps` launches with the inputs resident, and the HBM
+    fraction at those bytes."""
+    import numpy as np
+    import torch
+
+    from asteriy_2d_amd import native, problems, solver
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(13)
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps  # ms per launch
+
+    def unif(lo, hi):
+        return lo + (hi - lo) * torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+
+    res = {"items": n}
+    cp = torch.exp(unif(-3.0, 3.0))
+    u = cp * torch.exp(unif(-6.0, 2.5))
+    out = torch.empty((3, n), dtype=torch.float64, device=dev)
+    tr = timed(lambda: solver.mc_rates(u, cp, out=out, stream=stream))
+    res["rates"] = {"gitems": round(n / tr / 1e6, 3), "hbm_frac": round(40.0 * n / (tr * 1e-3) / 8e12, 4)}
+    del u, cp, out
+    g = problems.wind_grid(2048, 1024)
+    ci = torch.randint(0, g.N, (n,), generator=gen, device=dev).double()
+    cj = torch.randint(0, g.M, (n,), generator=gen, device=dev).double()
+    x0, y0 = g.x_min + ci * g.dx, g.y_min + cj * g.dy
+    r = y0 + unif(-0.45, 0.45) * g.dy
+    phi = unif(0.0, 2.0 * np.pi)
+    P = torch.stack([x0, y0, x0 + unif(-0.45, 0.45) * g.dx, r * torch.cos(phi), r * torch.sin(phi),
+                     torch.randn(n, generator=gen, device=dev, dtype=torch.float64),
+                     torch.randn(n, generator=gen, device=dev, dtype=torch.float64),
+                     torch.randn(n, generator=gen, device=dev, dtype=torch.float64)]).contiguous()
+    del ci, cj, x0, y0, r, phi
+    t = torch.empty(n, dtype=torch.float64, device=dev)
+    mode = torch.empty(n, dtype=torch.int32, device=dev)
+    hit = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def exits():
+        native.call("ast_mc_cell_exit_f64", native.grid_ref(g), n, solver._ptr(P), solver._ptr(t),
+                    solver._ptr(mode), solver._ptr(hit), solver._stream(stream))
+    te = timed(exits)
+    res["cell_exit"] = {"gitems": round(n / te / 1e6, 3),
+                        "hbm_frac": round(80.0 * n / (te * 1e-3) / 8e12, 4)}
+    del P, t, mode, hit
     return res
 
 
@@ -394,14 +456,16 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
     ap.add_argument("--profile", choices=["f32", "f64", "f64src", "f64pulse", "exact32",
-                                          "exact32l", "exact64", "batch", "flux", "mhd"],
+                                          "exact32l", "exact64", "batch", "flux", "mhd",
+                                          "mc"],
                     default=None,
                     help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
                          "step in FP32 / FP64 from the initial state (f64src: with neutral-H "
                          "planes, f64pulse: with add2's pulsed inflow on the Dirichlet side), "
                          "or the exact-solver step (FP32, FP32 with the learned guess, FP64); "
                          "or --steps steps of the 64-instance sweep (batch); or the "
-                         "Riemann-flux microbenchmark (flux), or its MHD fluxes alone (mhd)")
+                         "Riemann-flux microbenchmark (flux), or its MHD fluxes alone (mhd), or "
+                         "the Monte-Carlo helpers (mc)")
     ap.add_argument("--rows", type=int, default=0,
                     help="--profile: FP32 strip tiling, rows per strip or -S strips per band "
                          "(default: the autotuner's choice)")
@@ -454,6 +518,9 @@ def main(argv=None):
         return 0
     if a.profile == "mhd":
         log(f"mhd: {_mhd_microbenchmark(reps=K)}")
+        return 0
+    if a.profile == "mc":
+        log(f"mc: {_mc_microbenchmark(reps=K)}")
         return 0
     if a.profile in ("exact32", "exact32l", "exact64"):
         f32 = a.profile != "exact64"

@@ -35,6 +35,12 @@ int main(int argc, char** argv) {
       out[i] = asteriy::mc::velosity_1(in[i], in[n + i]);
       out[n + i] = asteriy::mc::velosity_2(in[i], in[n + i]);
       out[2 * n + i] = asteriy::mc::velosity_3(in[i], in[n + i]);
+      double v1, v2, v3;  // the fused form: the same bits
+      asteriy::mc::velosities(in[i], in[n + i], v1, v2, v3);
+      if (!(v1 == out[i] || (v1 != v1 && out[i] != out[i])) ||
+          !(v2 == out[n + i] || (v2 != v2 && out[n + i] != out[n + i])) ||
+          !(v3 == out[2 * n + i] || (v3 != v3 && out[2 * n + i] != out[2 * n + i])))
+        return 6;
     }
   } else if (!strcmp(argv[1], "exit") && argc == 6) {
     const size_t n = in.size() / 8;
