@@ -21,13 +21,15 @@
 #include <vector>
 
 #include "../../include/asteriy.h"
+#include "step_sources.h"
 #include "time_slots.h"
 
 int ast_fail(int code, const char* fmt, ...);
 int ast_hip_fail(hipError_t e, const char* where);
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, const ast::StepTime* tm, void* stream);
+                      ast_stats_t* stats, int krows, const ast::StepTime* tm,
+                      const ast::StepSources* src, void* stream);
 int ast_funk_time_fold(double* T, int last, int nslots, uint32_t* flags, void* stream);
 int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, double* s2,
                       double* u2, double* T, const double* T_do, const double* TT, int method,
@@ -70,9 +72,12 @@ struct ast_sim {
   long long runs;           // runs launched in learned mode
   int mode_init[2];         // modes of runs 0 and 1
   int mode_force;           // -1: automatic; 0 / 1: every run plain / learned
-  // neutral-H source planes (FP64 only): device copies owned by the
-  // simulation, null when not set
+  // neutral-H source planes, device copies owned by the simulation (null
+  // when not set): FP64 simulations keep the reference's planes (src.nn2,
+  // src.nn3), FP32 ones their FP32 roundings (src32.xy, src32.e), which the
+  // FP32 step reads with the same results
   ast_sources_t src;
+  ast::StepSources src32;
 };
 typedef struct ast_sim ast_sim_t;
 
@@ -81,11 +86,21 @@ static size_t elem(const ast_sim_t* s) { return s->precision ? sizeof(float) : s
 // steps per captured graph (even: a replay ends on the plane it starts from)
 constexpr int kChunk = 20;
 
-// the learned mode applies: FP32 exact solver with a learned guess and no
-// face window (the learned kernels need full waves)
+static bool has_sources(const ast_sim_t* S) {
+  return S->src.nn2 || S->src.nn3 || S->src32.xy || S->src32.e;
+}
+static const ast_sources_t* sources64(const ast_sim_t* S) {
+  return S->src.nn2 || S->src.nn3 ? &S->src : nullptr;
+}
+static const ast::StepSources* sources32(const ast_sim_t* S) {
+  return S->src32.xy || S->src32.e ? &S->src32 : nullptr;
+}
+
+// the learned mode applies: FP32 exact solver with a learned guess, no face
+// window (the learned kernels need full waves) and no source planes
 static bool learned_mode(const ast_sim_t* S) {
   return S->precision == 1 && S->method == AST_EXACT && S->mlp != nullptr &&
-         S->g.ho_lo == -INFINITY && S->g.ho_hi == INFINITY;
+         S->g.ho_lo == -INFINITY && S->g.ho_hi == INFINITY && !has_sources(S);
 }
 
 static ast::StepTime step_time(ast_sim_t* S, int j, int mode) {
@@ -118,13 +133,13 @@ static int launch_steps(ast_sim_t* S, int from, int n, int mode) {
       rc = ast_step_f64_rows(&S->g, (const double*)s, (const double*)u, (double*)s2,
                              (double*)u2, T, T + 1, T + 2, S->method,
                              S->collect ? S->stats : nullptr,
-                             S->rows64, S->src.nn2 || S->src.nn3 ? &S->src : nullptr, S->stream);
+                             S->rows64, sources64(S), S->stream);
       if (rc == AST_OK) rc = ast_funk_time(T, T + 1, T + 2, (int*)(T + 3), S->stream);
     } else {
       const ast::StepTime tm = step_time(S, j, mode);
       rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
                              T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
-                             &tm, S->stream);
+                             &tm, sources32(S), S->stream);
     }
     if (rc != AST_OK) return rc;
   }
@@ -217,6 +232,8 @@ extern "C" int ast_sim_destroy(ast_sim_t* S) {
   (void)hipHostFree(S->hflags);
   (void)hipFree(const_cast<double*>(S->src.nn2));
   (void)hipFree(const_cast<double*>(S->src.nn3));
+  (void)hipFree(const_cast<float*>(S->src32.xy));
+  (void)hipFree(const_cast<float*>(S->src32.e));
   for (int k = 0; k < 2; k++) (void)hipEventDestroy(S->run_ev[k]);
   (void)hipStreamDestroy(S->stream);
   delete S;
@@ -225,28 +242,57 @@ extern "C" int ast_sim_destroy(ast_sim_t* S) {
 
 // Neutral-H source planes from host memory (the reference fills nn1..nn3 on
 // the host and copies them once, kernel.cu:3337-3339, 3453-3455): nn2 N*M
-// triples, nn3 N*M doubles, either may be NULL; both NULL removes them.
+// triples, nn3 N*M doubles, either may be NULL; both NULL removes them.  An
+// FP32 simulation keeps the planes' FP32 roundings (nn2's x and y, nn3): its
+// step rounds each plane value to FP32 before using it, so they give the
+// same results in 12 B per cell instead of 32.  The learned start is not
+// used while planes are set.
 extern "C" int ast_sim_set_sources(ast_sim_t* S, const double* nn2, const double* nn3,
                                    double kmom, double ken) {
   if (!S) return ast_fail(AST_ERR_ARG, "ast_sim_set_sources: null sim");
-  if (S->precision != 0 && (nn2 || nn3))
-    return ast_fail(AST_ERR_ARG, "ast_sim_set_sources: source planes need an FP64 simulation");
   CK(hipStreamSynchronize(S->stream));
   drop_graph(S);  // captured runs hold the old planes
   CK(hipFree(const_cast<double*>(S->src.nn2)));
   CK(hipFree(const_cast<double*>(S->src.nn3)));
+  CK(hipFree(const_cast<float*>(S->src32.xy)));
+  CK(hipFree(const_cast<float*>(S->src32.e)));
   S->src = ast_sources_t{nullptr, nullptr, kmom, ken};
+  S->src32 = ast::StepSources{nullptr, nullptr, nullptr, nullptr, kmom, ken};
+  const size_t n = S->cells;
+  if (S->precision == 1) {
+    std::vector<float> h;
+    if (nn2) {
+      h.resize(2 * n);
+      for (size_t k = 0; k < n; k++) {
+        h[2 * k] = (float)nn2[3 * k];
+        h[2 * k + 1] = (float)nn2[3 * k + 1];
+      }
+      float* d = nullptr;
+      CK(hipMalloc(&d, 2 * n * sizeof(float)));
+      S->src32.xy = d;
+      CK(hipMemcpy(d, h.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (nn3) {
+      h.resize(n);
+      for (size_t k = 0; k < n; k++) h[k] = (float)nn3[k];
+      float* d = nullptr;
+      CK(hipMalloc(&d, n * sizeof(float)));
+      S->src32.e = d;
+      CK(hipMemcpy(d, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return AST_OK;
+  }
   if (nn2) {
     double* d = nullptr;
-    CK(hipMalloc(&d, S->cells * 3 * sizeof(double)));
+    CK(hipMalloc(&d, n * 3 * sizeof(double)));
     S->src.nn2 = d;
-    CK(hipMemcpy(d, nn2, S->cells * 3 * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(d, nn2, n * 3 * sizeof(double), hipMemcpyHostToDevice));
   }
   if (nn3) {
     double* d = nullptr;
-    CK(hipMalloc(&d, S->cells * sizeof(double)));
+    CK(hipMalloc(&d, n * sizeof(double)));
     S->src.nn3 = d;
-    CK(hipMemcpy(d, nn3, S->cells * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(d, nn3, n * sizeof(double), hipMemcpyHostToDevice));
   }
   return AST_OK;
 }
@@ -426,11 +472,10 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
     return S->precision == 1
                ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                    (float*)u2, S->scal, S->scal + 1, S->method, S->mlp, nullptr,
-                                   kr, &tm, S->stream)
+                                   kr, &tm, sources32(S), S->stream)
                : ast_step_f64_rows(&S->g, (const double*)sp, (const double*)up, (double*)s2,
                                    (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
-                                   nullptr, kr,
-                                   S->src.nn2 || S->src.nn3 ? &S->src : nullptr, S->stream);
+                                   nullptr, kr, sources64(S), S->stream);
   };
   auto valid = [&](int kr) {
     if (S->precision == 0) return kr >= 1 && kr <= 4096;

@@ -32,6 +32,7 @@
 // method in step_f32_hll.hip, step_f32_exact.hip, step_f32_learned.hip and
 // step_f32_detect.hip (step_f32_kernel.hpp).
 #include "step_f32_kernel.hpp"
+#include "step_sources.h"
 
 using namespace ast;
 using namespace ast::f32;
@@ -58,7 +59,8 @@ extern "C" int ast_get_tile_rows(void) { return g_krows; }
 // two-shock faces in *tm->strong.
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
                       double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, const StepTime* tm, void* stream) {
+                      ast_stats_t* stats, int krows, const StepTime* tm,
+                      const StepSources* src, void* stream) {
   int rc = ast_grid_check(g);
   if (rc != AST_OK) return rc;
   if (!s || !u || !s2 || !u2 || !T || !T_do)
@@ -82,8 +84,16 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   const Consts c = make_consts(*g);
   const ExactCtx ex = make_exact_ctx(g->gamma, mlp);
   const bool window = !(g->ho_lo == -INFINITY && g->ho_hi == INFINITY);
+  const bool has_src = src != nullptr && (src->nn2 != nullptr || src->nn3 != nullptr ||
+                                          src->xy != nullptr || src->e != nullptr);
+  const bool f32_src = has_src && (src->xy != nullptr || src->e != nullptr);
+  const SrcF sf = has_src ? SrcF{f32_src ? nullptr : src->nn2, f32_src ? nullptr : src->nn3,
+                                 (const float2*)src->xy, src->e, (float)src->kmom,
+                                 (float)src->ken}
+                          : SrcF{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f};
   StepArgs a{*g, c, ex, window, (const float2*)s, (const float2*)u, (float2*)s2, (float2*)u2,
-             T, T_do, kt, stats, krows, (hipStream_t)stream};
+             T, T_do, kt, stats, krows, (hipStream_t)stream, has_src ? &sf : nullptr};
+  if (has_src) mlp = nullptr;  // the learned start is not served with source planes
   switch (method) {
     case AST_HLL: return step_launch<AST_HLL>(a);
     case AST_HLLC: return step_launch<AST_HLLC>(a);
@@ -103,5 +113,17 @@ extern "C" int ast_step_f32(const ast_grid_t* g, const float* s, const float* u,
                             float* u2, double* T, const double* T_do, double* TT, int method,
                             const ast_mlp_t* mlp, ast_stats_t* stats, void* stream) {
   (void)TT;
-  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, nullptr, stream);
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, nullptr,
+                           nullptr, stream);
+}
+
+extern "C" int ast_step_f32_src(const ast_grid_t* g, const float* s, const float* u, float* s2,
+                                float* u2, double* T, const double* T_do, double* TT,
+                                int method, const ast_sources_t* src, ast_stats_t* stats,
+                                void* stream) {
+  (void)TT;
+  const StepSources ss = src ? StepSources{src->nn2, src->nn3, nullptr, nullptr, src->kmom, src->ken}
+                             : StepSources{nullptr, nullptr, nullptr, nullptr, 0.0, 0.0};
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, nullptr, stats, g_krows, nullptr,
+                           src ? &ss : nullptr, stream);
 }

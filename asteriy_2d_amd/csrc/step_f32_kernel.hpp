@@ -28,6 +28,7 @@ struct StepArgs {
   ast_stats_t* stats;
   int krows;
   hipStream_t stream;
+  const SrcF* src;  // neutral-H planes, or null
 };
 
 // launches one method's kernel for a.g's geometry and a.window; defined
@@ -44,8 +45,10 @@ extern template int step_launch<kExactDetect>(const StepArgs&);
 
 #ifdef AST_STEP_F32_KERNELS
 
-// STATS: the variant that counts the exact solver's work into *stats
-template <int METHOD, int GEOM, bool WINDOW, bool STATS>
+// STATS: the variant that counts the exact solver's work into *stats; SRC:
+// the variants with add_MK's neutral-hydrogen source planes (kSrcF64, the
+// reference's planes; kSrcF32, FP32 copies of them)
+template <int METHOD, int GEOM, bool WINDOW, bool STATS, int SRC>
 __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactCtx ex,
                                                   const float2* __restrict__ s,
                                                   const float2* __restrict__ u,
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   float2* __restrict__ u2, double* T,
                                                   const double* __restrict__ T_do, KTime kt,
                                                   ast_stats_t* stats, int nbands, int krows,
-                                                  int nwg) {
+                                                  int nwg, SrcF src) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
   const double tdo = step_dt_load(T_do, kt);
@@ -67,7 +70,8 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int band, m0, m1;
   strip_rows(g.M, nbands, krows, xcd_tile(blockIdx.x, nwg), wave, band, m0, m1);
-  tmin = strip_of_wave<METHOD, GEOM, WINDOW>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st);
+  tmin = strip_of_wave<METHOD, GEOM, WINDOW, SRC>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st,
+                                                  src);
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
@@ -76,7 +80,8 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
 template <int METHOD, int GEOM, bool WINDOW>
 int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
                       const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
-                      const KTime& kt, ast_stats_t* stats, int krows, hipStream_t stream) {
+                      const KTime& kt, ast_stats_t* stats, int krows, hipStream_t stream,
+                      const SrcF* src) {
   // krows > 0: strips of krows rows; krows = -S: S strips per band (strip_rows)
   const int nbands = (g.N + WOUT - 1) / WOUT;
   long long nwg;
@@ -88,15 +93,39 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
   }
   if (nwg > 0x7fffffffLL / NT) return ast_fail(AST_ERR_SIZE, "ast_step_f32: grid too large");
   const unsigned grid = (unsigned)nwg;
-  if (is_exact(METHOD) && stats != nullptr) {
+  const SrcF none{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f};
+  constexpr bool kSrc = METHOD != kExactLearned && METHOD != kExactDetect;
+  if (src != nullptr) {
+    if constexpr (kSrc) {
+      const bool f32 = src->xy != nullptr || src->e != nullptr;
+      const bool counting = is_exact(METHOD) && stats != nullptr;
+#define AST_LAUNCH_SRC(STATS_, SRC_)                                                           \
+  hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, STATS_, SRC_>), dim3(grid), dim3(NT), \
+                     0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,    \
+                     (int)nwg, *src)
+      if (counting) {
+        if constexpr (is_exact(METHOD)) {
+          if (f32) AST_LAUNCH_SRC(true, kSrcF32);
+          else AST_LAUNCH_SRC(true, kSrcF64);
+        }
+      } else if (f32) {
+        AST_LAUNCH_SRC(false, kSrcF32);
+      } else {
+        AST_LAUNCH_SRC(false, kSrcF64);
+      }
+#undef AST_LAUNCH_SRC
+    } else {
+      return ast_fail(AST_ERR_ARG, "ast_step_f32: source planes with the learned start");
+    }
+  } else if (is_exact(METHOD) && stats != nullptr) {
     if constexpr (is_exact(METHOD))
-      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true>), dim3(grid), dim3(NT), 0,
+      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true, kSrcNone>), dim3(grid), dim3(NT), 0,
                          stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
-                         (int)nwg);
+                         (int)nwg, none);
   } else {
-    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false>), dim3(grid), dim3(NT), 0,
-                       stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
-                       (int)nwg);
+    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false, kSrcNone>), dim3(grid), dim3(NT),
+                       0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
+                       (int)nwg, none);
   }
   AST_LAUNCH_CHECK("ast_step_f32 launch");
   return AST_OK;
@@ -107,7 +136,7 @@ int step_launch(const StepArgs& a) {
   const bool axi = a.g.geometry == AST_AXISYMMETRIC;
 #define AST_LAUNCH(G, W)                                                                     \
   launch_f32<METHOD, G, W>(a.g, a.c, a.ex, a.s, a.u, a.s2, a.u2, a.T, a.T_do, a.kt, a.stats, \
-                           a.krows, a.stream)
+                           a.krows, a.stream, a.src)
   if constexpr (METHOD == kExactLearned || METHOD == kExactDetect) {  // without a window only
     return axi ? AST_LAUNCH(AST_AXISYMMETRIC, false) : AST_LAUNCH(AST_PLANAR, false);
   } else {

@@ -3,6 +3,7 @@
 // (batch.hip).  See step_f32.hip for the scheme.
 #pragma once
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "exact_f32.hpp"
@@ -285,7 +286,56 @@ AST_DEV bool frozen_possible(const ast_grid_t& g, int n0, int n1, int m0, int m1
   return x1 >= g.frozen_x0 - ax && x0 <= g.frozen_x0 + ax && y1 >= -ay && y0 <= ay;
 }
 
+// add_MK's neutral-hydrogen source planes, either of a pair may be null:
+// the reference's FP64 planes (nn2 three doubles per cell, x and y used; nn3
+// one), or FP32 copies of them (xy: nn2's x and y; e: nn3), each value the
+// FP32 rounding of the plane's -- the step rounds every plane value to FP32
+// before it uses it, so the copies give the same bits in 12 B per cell
+// instead of 32
+struct SrcF {
+  const double* nn2;
+  const double* nn3;
+  const float2* xy;
+  const float* e;
+  float kmom, ken;
+};
+
+// source kinds of the step kernels: none, the FP64 planes, the FP32 copies
+constexpr int kSrcNone = 0, kSrcF64 = 1, kSrcF32 = 2;
+
+// one cell's plane values, {nn2.x, nn2.y, nn3} (0 where a plane is absent),
+// held in the planes' own type until used
+template <int SRC>
+struct SrcV {
+  using T = typename std::conditional<SRC == kSrcF64, double, float>::type;
+  T x, y, e;
+};
+
+// the plane values of cell (n, m), clamped into the grid: lanes and rows
+// outside it read a cell inside it (their results are not stored)
+template <int SRC>
+AST_DEV SrcV<SRC> src_load(const SrcF& s, int n, int m, int N, int M) {
+  const size_t k = (size_t)min(max(m, 0), M - 1) * N + min(max(n, 0), N - 1);
+  SrcV<SRC> v{0, 0, 0};
+  if constexpr (SRC == kSrcF64) {
+    if (s.nn2) {  // wave-uniform
+      v.x = s.nn2[3 * k];
+      v.y = s.nn2[3 * k + 1];
+    }
+    if (s.nn3) v.e = s.nn3[k];
+  } else if constexpr (SRC == kSrcF32) {
+    if (s.xy) {
+      const float2 q = s.xy[k];
+      v.x = q.x;
+      v.y = q.y;
+    }
+    if (s.e) v.e = s.e[k];
+  }
+  return v;
+}
+
 struct StripArgs {
+  SrcF src;
   int n, m0, m1;
   bool cols_in, out_col, xvote, ho_x, ho_y;
   // fast strips of the first / last band: the ghost lanes (gmask) apply
@@ -342,7 +392,7 @@ struct Row {
 // the whole row, the frozen tests of kEdge.
 constexpr int kEdge = 0, kFast = 1, kFastX = 2, kFastE = 3;
 
-template <int METHOD, int GEOM, bool WINDOW, int PATH, class ST>
+template <int METHOD, int GEOM, bool WINDOW, int PATH, int SRC, class ST>
 AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, const ExactCtx& ex,
                        const float2* __restrict__ s, const float2* __restrict__ u,
                        float2* __restrict__ s2, float2* __restrict__ u2, const StripArgs& a,
@@ -440,6 +490,12 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     tyl = a.tdof * frcp(a.ymin + (m0 + (int)(threadIdx.x & 63)) * a.dyf);
   Q pend = w[0];
   unsigned pend_vo = kOffDrop;
+  // source planes: row m's values are fetched with row m-1's state fetch,
+  // so that they arrive in order with it (memory operations retire in
+  // order: a fetch issued where the values are used would make that use
+  // wait for the state fetch issued before it)
+  SrcV<SRC> scur{0, 0, 0};
+  if constexpr (SRC != kSrcNone) scur = src_load<SRC>(a.src, n, m0, N, M);
 
   auto row = [&](auto tag, int m) {
     using R = decltype(tag);
@@ -447,6 +503,10 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     const Q& q1 = w[R::w1];
     const Q& q2 = w[R::w2];
     w[R::w3] = LD(min(m + 3, mlast));
+    SrcV<SRC> snext{0, 0, 0};
+    // (the last row of the strip re-reads its own values, which are cached,
+    // rather than the next strip's)
+    if constexpr (SRC != kSrcNone) snext = src_load<SRC>(a.src, n, min(m + 1, m1 - 1), N, M);
     bstore(pend.r, pend.p, ws, pend_vo, 0);
     bstore(pend.vx, pend.vy, wu, pend_vo, 0);
     // keep the fetch of row m+3 here, a row ahead of its use (the
@@ -502,8 +562,8 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     const CellFlux cS = in_cell_frame<ST1>(Gs, q0.vy, q0.vx);
     float dr_ = a.ax * (Fe.m - Fw.m) + a.ay * (Gn.m - Gs.m);
     float de = a.ax * (cE.e - cW.e) + a.ay * (cN.e - cS.e);
-    const float d1x = a.ax * (cE.n - cW.n) + a.ay * (cN.t - cS.t);
-    const float d1y = a.ax * (cE.t - cW.t) + a.ay * (cN.n - cS.n);
+    float d1x = a.ax * (cE.n - cW.n) + a.ay * (cN.t - cS.t);
+    float d1y = a.ax * (cE.t - cW.t) + a.ay * (cN.n - cS.n);
     G[R::e1] = Gn;
     if constexpr (GEOM == AST_AXISYMMETRIC) {
       // KorolkovSolver.cu:13-32: -(T_do/y) v {rho, rho u, rho v, E + p}; in
@@ -515,6 +575,19 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
       const float f = ty * q0.vy;
       dr_ = fmaf(-f, q0.r, dr_);
       de = fmaf(-f, c.gig1 * q0.p, de);
+    }
+    if constexpr (SRC != kSrcNone) {
+      // add_MK's neutral-hydrogen terms (kernel.cu:2495-2498), before the
+      // floors: momentum T_do (rho kmom) nn2.{x, y}, energy T_do ken nn3;
+      // seen from the cell the energy change loses u.(the momentum change)
+      const float tk = a.tdof * (q0.r * a.src.kmom);
+      const float mx = tk * (float)scur.x;
+      const float my = tk * (float)scur.y;
+      const float se = (a.tdof * a.src.ken) * (float)scur.e;
+      scur = snext;
+      d1x += mx;
+      d1y += my;
+      de += se - fmaf(q0.vx, mx, q0.vy * my);
     }
     // kernel.cu:3060-3064: a non-positive density keeps the old one, while
     // the momentum change stays (the reference divides rho u + dm by the old
@@ -632,13 +705,15 @@ AST_DEV void strip_rows(int M, int nbands, int tile, int wg, int wave, int& band
 // The strip of one wave given its band and rows (strip_rows): fills
 // StripArgs and runs the fast or the boundary variant.  Returns the wave's
 // dt candidate (wave-uniform).
-template <int METHOD, int GEOM, bool WINDOW, class ST>
+template <int METHOD, int GEOM, bool WINDOW, int SRC = kSrcNone, class ST>
 AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCtx& ex,
                              const float2* __restrict__ s, const float2* __restrict__ u,
                              float2* __restrict__ s2, float2* __restrict__ u2, double tdo,
-                             int band, int m0, int m1, ST& st) {
+                             int band, int m0, int m1, ST& st,
+                             const SrcF& src = SrcF{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f}) {
   const int lane = threadIdx.x & 63;
   StripArgs a;
+  a.src = src;
   a.m0 = m0;
   a.m1 = m1;
   a.n = band * WOUT + lane - 2;
@@ -689,9 +764,9 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
     const bool rows_in = a.m0 - 2 >= 0 && a.m1 + 1 < g.M;
     if (fast_rows && rows_in && !cw.frozen) {
       if (a.cols_in)
-        run_strip<METHOD, GEOM, WINDOW, kFast>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+        run_strip<METHOD, GEOM, WINDOW, kFast, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
       else
-        run_strip<METHOD, GEOM, WINDOW, kFastX>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+        run_strip<METHOD, GEOM, WINDOW, kFastX, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
     } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0) {
       // the y-ghost rules of fetch_cell as row rules (only the first or last
       // strips of a band hold ghost rows)
@@ -706,9 +781,9 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
         a.yhi[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 0) ? yc : 3.40282347e38f;
         for (int j = 0; j < 4; j++) a.yq[k][j] = (float)g.bc_state[ys][j];
       }
-      run_strip<METHOD, GEOM, WINDOW, kFastE>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      run_strip<METHOD, GEOM, WINDOW, kFastE, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
     } else {
-      run_strip<METHOD, GEOM, WINDOW, kEdge>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      run_strip<METHOD, GEOM, WINDOW, kEdge, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
     }
   }
   // only output lanes vote (halo lanes may hold garbage speeds)

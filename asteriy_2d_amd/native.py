@@ -46,6 +46,8 @@ SIGNATURES = {
     "ast_step_f64_src": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
                                   ctypes.POINTER(CSources), _vp, _vp]),
     "ast_step_f32": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp]),
+    "ast_step_f32_src": (_c_int, [_gp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
+                                  ctypes.POINTER(CSources), _vp, _vp]),
     "ast_funk_time": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ast_conservation_work_doubles": (_c_size, []),
     "ast_conservation_sums": (_c_int, [_gp, _vp, _vp, _c_int, _vp, _vp]),

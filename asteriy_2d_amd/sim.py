@@ -132,7 +132,8 @@ class Simulation:
         """Neutral-H source planes of add_MK (kernel.cu:2495-2498) for every
         later step: nn2 (M, N, 3), nn3 (M, N) host float64 arrays (either may
         be None); the defaults are the reference's n_H/Kn and 0.0*n_H/Kn
-        (Header.h:84-85).  FP64 simulations only."""
+        (Header.h:84-85).  An FP32 simulation with planes runs the exact
+        solver without the learned start."""
         a2 = None if nn2 is None else np.ascontiguousarray(nn2, dtype=np.float64)
         a3 = None if nn3 is None else np.ascontiguousarray(nn3, dtype=np.float64)
         if a2 is not None and a2.shape != (self.grid.M, self.grid.N, 3):

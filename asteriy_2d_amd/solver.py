@@ -90,7 +90,7 @@ def Kernel_TVD(grid, s, u, s2, u2, T, T_do, TT, method: int, stats=None, mlp=Non
     of the reference's step), with `sources` the neutral-H terms of add_MK
     (ast_step_f64_src); FP32 tensors run the fast path (ast_step_f32),
     optionally seeding the exact solver with a learned guess (`mlp`, see
-    mlp.py)."""
+    mlp.py), or with `sources` (ast_step_f32_src, no learned guess)."""
     import torch
     cells = grid.N * grid.M
     dt = s.dtype
@@ -110,9 +110,14 @@ def Kernel_TVD(grid, s, u, s2, u2, T, T_do, TT, method: int, stats=None, mlp=Non
         else:
             native.call("ast_step_f64", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
                         _ptr(T_do), _ptr(TT), int(method), _ptr(stats), _stream(stream))
+    elif sources is not None:
+        if mlp is not None:
+            raise ValueError("the FP32 step with source planes runs without the learned guess")
+        cs = sources.c(cells)
+        native.call("ast_step_f32_src", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
+                    _ptr(T_do), _ptr(TT), int(method), ctypes.byref(cs), _ptr(stats),
+                    _stream(stream))
     else:
-        if sources is not None:
-            raise ValueError("the neutral-H sources are served by the FP64 step only")
         native.call("ast_step_f32", gref, _ptr(s), _ptr(u), _ptr(s2), _ptr(u2), _ptr(T),
                     _ptr(T_do), _ptr(TT), int(method),
                     _ptr(mlp.buffer) if mlp is not None else None, _ptr(stats),

@@ -62,7 +62,7 @@ def _timed(runner, steps: int, stream):
 def _kernel_ms(grid, precision, method, stream, krows=None, reps=50, sources=None):
     """Average duration of the step kernel alone: `reps` back-to-back launches
     on `stream` between two HIP events (the queue stays full, so the launch
-    gaps are hidden).  sources: neutral-H planes for the FP64 step."""
+    gaps are hidden).  sources: neutral-H planes."""
     import torch
 
     from asteriy_2d_amd import native, problems, solver
@@ -455,13 +455,16 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
-    ap.add_argument("--profile", choices=["f32", "f64", "f64src", "f64pulse", "exact32",
+    ap.add_argument("--profile", choices=["f32", "f32src", "f32src64", "f64", "f64src",
+                                          "f64pulse", "exact32",
                                           "exact32l", "exact64", "batch", "flux", "mhd",
                                           "mc"],
                     default=None,
                     help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
-                         "step in FP32 / FP64 from the initial state (f64src: with neutral-H "
-                         "planes, f64pulse: with add2's pulsed inflow on the Dirichlet side), "
+                         "step in FP32 / FP64 from the initial state (f32src / f64src: with "
+                         "neutral-H planes in the runtime, f32src64: the FP32 step's C entry "
+                         "point with the FP64 planes, f64pulse: with add2's pulsed inflow on "
+                         "the Dirichlet side), "
                          "or the exact-solver step (FP32, FP32 with the learned guess, FP64); "
                          "or --steps steps of the 64-instance sweep (batch); or the "
                          "Riemann-flux microbenchmark (flux), or its MHD fluxes alone (mhd), or "
@@ -545,12 +548,22 @@ def main(argv=None):
             n_, m_ = (int(v) for v in a.grid.lower().split("x"))
             p = problems.wind(n_, m_, method=HLLC, steps=K)
         gp = p.grid
+        if a.profile == "f32src64":
+            from asteriy_2d_amd import solver
+            rng = np.random.default_rng(9)
+            dev = torch.device("cuda:0")
+            nn2 = torch.from_numpy(rng.uniform(-1e-3, 1e-3, (gp.M, gp.N, 3))).to(dev)
+            nn3 = torch.from_numpy(rng.uniform(-1e-4, 1e-4, (gp.M, gp.N))).to(dev)
+            _kernel_ms(gp, "f32", HLLC, torch.cuda.current_stream(), krows=a.rows or None,
+                       reps=K, sources=solver.NeutralSources(nn2, nn3, ken=0.02))
+            log(f"step_{a.profile} HLLC {gp.N}x{gp.M}: {K} steps")
+            return 0
         if a.profile == "f64pulse":
             from asteriy_2d_amd import BC_DIRICHLET_PULSED, PULSE_ADD2
             gp = gp.replace(bc=(gp.bc[0], BC_DIRICHLET_PULSED, gp.bc[2], gp.bc[3]),
                             bc_pulse=PULSE_ADD2)
-        sim = Simulation(gp, HLLC, "f32" if a.profile == "f32" else "f64", p.s, p.u)
-        if a.profile == "f64src":
+        sim = Simulation(gp, HLLC, "f32" if a.profile in ("f32", "f32src") else "f64", p.s, p.u)
+        if a.profile in ("f32src", "f64src"):
             rng = np.random.default_rng(9)
             sim.set_sources(rng.uniform(-1e-3, 1e-3, (gp.M, gp.N, 3)),
                             rng.uniform(-1e-4, 1e-4, (gp.M, gp.N)), ken=0.02)
@@ -616,6 +629,26 @@ def main(argv=None):
             "neutral_sources_hbm_frac": round(96.0 * cells / (ksrc * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "pulsed_inflow_us": round(kpul * 1e3, 2)}
         log(f"FP64 step variants: {extra['fp64_step_variants']}")
+        # the FP32 step with the same planes, kernel alone at the headline
+        # tiling: as the runtime runs it (the planes' FP32 roundings, 12 more
+        # bytes per cell on 32) and through ast_step_f32_src with the FP64
+        # planes (32 more bytes per cell)
+        h2 = rng.uniform(-1e-3, 1e-3, (g.M, g.N, 3))
+        h3 = rng.uniform(-1e-4, 1e-4, (g.M, g.N))
+        with Simulation(g, HLLC, "f32", p.s, p.u) as ss:
+            ss.set_sources(h2, h3, ken=0.02)
+            k32r = ss.autotune([rows], reps=200)[1] * 1e-3
+        nn2, nn3 = torch.from_numpy(h2).to(dev), torch.from_numpy(h3).to(dev)
+        k32s = _kernel_ms(g, "f32", HLLC, torch.cuda.current_stream(), krows=rows, reps=100,
+                          sources=solver.NeutralSources(nn2, nn3, ken=0.02))
+        del nn2, nn3, h2, h3
+        extra["fp32_step_variants"] = {
+            "plain_us": round(kms32 * 1e3, 2),
+            "neutral_sources_runtime_us": round(k32r * 1e3, 2),
+            "neutral_sources_runtime_hbm_frac": round(44.0 * cells / (k32r * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "neutral_sources_fp64_planes_us": round(k32s * 1e3, 2),
+            "neutral_sources_fp64_planes_hbm_frac": round(64.0 * cells / (k32s * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"FP32 step variants: {extra['fp32_step_variants']}")
         sr = p.s.astype(np.float32).astype(np.float64)
         ur = p.u.astype(np.float32).astype(np.float64)
         _, rfl, _, _, _ = _run_sim(g, HLLC, "f64", sr, ur, K)

@@ -306,6 +306,14 @@ int ast_step_f64_src(const ast_grid_t* g, const double* s, const double* u,
                      double* s2, double* u2, double* T, const double* T_do,
                      double* TT, int method, const ast_sources_t* src,
                      ast_stats_t* stats, void* stream);
+/* The same source terms (kernel.cu:2495-2498) behind the FP32 fast step
+ * (ast_step_f32 on FP32 state; the planes stay the reference's FP64 ones,
+ * each value rounded to FP32 before use; the exact solver runs without the
+ * learned start).  src == NULL is ast_step_f32 without a learned guess. */
+int ast_step_f32_src(const ast_grid_t* g, const float* s, const float* u,
+                     float* s2, float* u2, double* T, const double* T_do,
+                     double* TT, int method, const ast_sources_t* src,
+                     ast_stats_t* stats, void* stream);
 int ast_flux_f64_guess(size_t n, const double* in, double n1, double n2,
                        double rad, double gamma, double krit, const float* guess,
                        double* out, ast_stats_t* stats, int* info, void* stream);
@@ -389,9 +397,10 @@ int ast_sim_collect_stats(ast_sim_t* sim, int on);
  * for the run in flight); the first two runs after a create or load take
  * the plain solver (or the modes the checkpoint recorded). */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
-/* Neutral-H source planes for every later step (ast_step_f64_src), copied
- * from host memory: nn2 N*M triples, nn3 N*M doubles, either may be NULL,
- * both NULL removes them.  FP64 simulations only (else AST_ERR_ARG). */
+/* Neutral-H source planes for every later step (ast_step_f64_src,
+ * ast_step_f32_src), copied from host memory: nn2 N*M triples, nn3 N*M
+ * doubles, either may be NULL, both NULL removes them.  An FP32 simulation
+ * with planes runs the exact solver without the learned start. */
 int ast_sim_set_sources(ast_sim_t* sim, const double* nn2, const double* nn3,
                         double kmom, double ken);
 /* The learned modes (0 plain solver, 1 learned start) the next two runs
