@@ -170,6 +170,10 @@ extern "C" int ast_batch_create(int count, const ast_grid_t* grids, int method, 
       return ast_fail(AST_ERR_ARG, "ast_batch_create: instance %d: the batch reconstructs every face "
                       "(ho_lo = -inf, ho_hi = +inf)", k);
     if ((rc = check_positive_bc(grids[k])) != AST_OK) return rc;
+    for (int j = 0; j < 4; j++)
+      if (grids[k].bc[j] == AST_BC_DIRICHLET_PULSED)
+        return ast_fail(AST_ERR_ARG, "ast_batch_create: instance %d: the batch has no simulation "
+                        "time for the pulsed inflow of side %d", k, j);
     if (!strip_fits(grids[k], 16))
       return ast_fail(AST_ERR_SIZE, "ast_batch_create: instance %d: %d columns exceed one buffer range",
                       k, grids[k].N);

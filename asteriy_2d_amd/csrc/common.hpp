@@ -43,6 +43,13 @@ AST_DEV double pulse_factor(const ast_grid_t& g, double TT) {
   return TT < g.bc_pulse[3] ? 1.0 + g.bc_pulse[0] * sin(TT * g.bc_pulse[1]) : 1.0;
 }
 
+// Does row m (ghost rows included) have its centre below the pulsed sides'
+// y_lim?  The sum is formed unfused, as the host forms it.
+__host__ __device__ __forceinline__ bool pulsed_row(const ast_grid_t& g, int m) {
+#pragma clang fp contract(off)
+  return g.y_min + m * g.dy < g.bc_pulse[2];
+}
+
 // Cell (n, m) of a state plane with the ghost rules of Kernel_TVD
 // (kernel.cu:2549-2698) and add_MK (kernel.cu:2328-2377).  Indices may lie
 // up to 2 cells outside the grid on at most one axis; anything else
@@ -72,7 +79,7 @@ AST_DEV void fetch_cell(const ast_grid_t& g, const P* __restrict__ s, const P* _
     so.y = g.bc_state[side][1];
     uo.x = g.bc_state[side][2];
     uo.y = g.bc_state[side][3];
-    if (bc == AST_BC_DIRICHLET_PULSED && g.y_min + m * g.dy < g.bc_pulse[2])
+    if (bc == AST_BC_DIRICHLET_PULSED && pulsed_row(g, m))
       so.x = (decltype(so.x))(g.bc_state[side][0] * pulse);
     return;
   }

@@ -27,9 +27,9 @@
 int ast_fail(int code, const char* fmt, ...);
 int ast_hip_fail(hipError_t e, const char* where);
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
-                      double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, const ast::StepTime* tm,
-                      const ast::StepSources* src, void* stream);
+                      double* T, const double* T_do, const double* TT, int method,
+                      const ast_mlp_t* mlp, ast_stats_t* stats, int krows,
+                      const ast::StepTime* tm, const ast::StepSources* src, void* stream);
 int ast_funk_time_fold(double* T, int last, int nslots, uint32_t* flags, void* stream);
 int ast_step_f64_rows(const ast_grid_t* g, const double* s, const double* u, double* s2,
                       double* u2, double* T, const double* T_do, const double* TT, int method,
@@ -138,8 +138,8 @@ static int launch_steps(ast_sim_t* S, int from, int n, int mode) {
     } else {
       const ast::StepTime tm = step_time(S, j, mode);
       rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
-                             T + 1, S->method, S->mlp, S->collect ? S->stats : nullptr, S->krows,
-                             &tm, sources32(S), S->stream);
+                             T + 1, T + 2, S->method, S->mlp, S->collect ? S->stats : nullptr,
+                             S->krows, &tm, sources32(S), S->stream);
     }
     if (rc != AST_OK) return rc;
   }
@@ -177,10 +177,6 @@ extern "C" int ast_sim_create(const ast_grid_t* g, int precision, int method, as
     return ast_fail(AST_ERR_ARG, "ast_sim_create: precision %d (0 = FP64, 1 = FP32)", precision);
   if (method < AST_HLL || method > AST_HLLC_A)
     return ast_fail(AST_ERR_ARG, "ast_sim_create: method %d", method);
-  for (int k = 0; k < 4 && precision == 1; k++)
-    if (g->bc[k] == AST_BC_DIRICHLET_PULSED)
-      return ast_fail(AST_ERR_ARG, "ast_sim_create: the pulsed inflow of side %d is served by the "
-                      "FP64 step only", k);
   if (ast_device_count() < 1) return ast_fail(AST_ERR_NOGPU, "ast_sim_create: no HIP device");
   ast_sim_t* S = new (std::nothrow) ast_sim_t();
   if (!S) return ast_fail(AST_ERR_SIZE, "ast_sim_create: out of host memory");
@@ -471,8 +467,8 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   auto step = [&](int kr) {
     return S->precision == 1
                ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
-                                   (float*)u2, S->scal, S->scal + 1, S->method, S->mlp, nullptr,
-                                   kr, &tm, sources32(S), S->stream)
+                                   (float*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
+                                   S->mlp, nullptr, kr, &tm, sources32(S), S->stream)
                : ast_step_f64_rows(&S->g, (const double*)sp, (const double*)up, (double*)s2,
                                    (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
                                    nullptr, kr, sources64(S), S->stream);

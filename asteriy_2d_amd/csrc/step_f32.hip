@@ -58,8 +58,8 @@ extern "C" int ast_get_tile_rows(void) { return g_krows; }
 // the learned start (tm->learned) or the plain solver; both flag strong
 // two-shock faces in *tm->strong.
 int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float* s2, float* u2,
-                      double* T, const double* T_do, int method, const ast_mlp_t* mlp,
-                      ast_stats_t* stats, int krows, const StepTime* tm,
+                      double* T, const double* T_do, const double* TT, int method,
+                      const ast_mlp_t* mlp, ast_stats_t* stats, int krows, const StepTime* tm,
                       const StepSources* src, void* stream) {
   int rc = ast_grid_check(g);
   if (rc != AST_OK) return rc;
@@ -82,6 +82,8 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
   rc = check_positive_bc(*g);
   if (rc != AST_OK) return rc;
   const Consts c = make_consts(*g);
+  if (c.pulsed && tm == nullptr && TT == nullptr)
+    return ast_fail(AST_ERR_ARG, "ast_step_f32: a pulsed Dirichlet side needs the time TT");
   const ExactCtx ex = make_exact_ctx(g->gamma, mlp);
   const bool window = !(g->ho_lo == -INFINITY && g->ho_hi == INFINITY);
   const bool has_src = src != nullptr && (src->nn2 != nullptr || src->nn3 != nullptr ||
@@ -92,7 +94,8 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
                                  (float)src->ken}
                           : SrcF{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f};
   StepArgs a{*g, c, ex, window, (const float2*)s, (const float2*)u, (float2*)s2, (float2*)u2,
-             T, T_do, kt, stats, krows, (hipStream_t)stream, has_src ? &sf : nullptr};
+             T, T_do, kt, stats, krows, (hipStream_t)stream, has_src ? &sf : nullptr,
+             c.pulsed && tm == nullptr ? TT : nullptr};
   if (has_src) mlp = nullptr;  // the learned start is not served with source planes
   switch (method) {
     case AST_HLL: return step_launch<AST_HLL>(a);
@@ -112,8 +115,7 @@ int ast_step_f32_rows(const ast_grid_t* g, const float* s, const float* u, float
 extern "C" int ast_step_f32(const ast_grid_t* g, const float* s, const float* u, float* s2,
                             float* u2, double* T, const double* T_do, double* TT, int method,
                             const ast_mlp_t* mlp, ast_stats_t* stats, void* stream) {
-  (void)TT;
-  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, mlp, stats, g_krows, nullptr,
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, TT, method, mlp, stats, g_krows, nullptr,
                            nullptr, stream);
 }
 
@@ -121,9 +123,8 @@ extern "C" int ast_step_f32_src(const ast_grid_t* g, const float* s, const float
                                 float* u2, double* T, const double* T_do, double* TT,
                                 int method, const ast_sources_t* src, ast_stats_t* stats,
                                 void* stream) {
-  (void)TT;
   const StepSources ss = src ? StepSources{src->nn2, src->nn3, nullptr, nullptr, src->kmom, src->ken}
                              : StepSources{nullptr, nullptr, nullptr, nullptr, 0.0, 0.0};
-  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, method, nullptr, stats, g_krows, nullptr,
-                           src ? &ss : nullptr, stream);
+  return ast_step_f32_rows(g, s, u, s2, u2, T, T_do, TT, method, nullptr, stats, g_krows,
+                           nullptr, src ? &ss : nullptr, stream);
 }

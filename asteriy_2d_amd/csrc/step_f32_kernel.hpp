@@ -29,6 +29,7 @@ struct StepArgs {
   int krows;
   hipStream_t stream;
   const SrcF* src;  // neutral-H planes, or null
+  const double* TT;  // simulation time for pulsed sides without kt (else unused)
 };
 
 // launches one method's kernel for a.g's geometry and a.window; defined
@@ -56,10 +57,27 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
                                                   float2* __restrict__ u2, double* T,
                                                   const double* __restrict__ T_do, KTime kt,
                                                   ast_stats_t* stats, int nbands, int krows,
-                                                  int nwg, SrcF src) {
+                                                  int nwg, SrcF src, const double* TT) {
   __shared__ double red[WPB];
   __shared__ uint32_t redi[WPB * 10];
   const double tdo = step_dt_load(T_do, kt);
+  // pulsed Dirichlet sides: the density factor at the time the reference's
+  // step reads (TT after the previous step's funk_time).  With the runtime's
+  // slots (kt), workgroup 0 advances T[2] during this step, so the step
+  // takes the time the previous step of its run used (word 2 of that step's
+  // slot set) plus its own dt -- the sum T[2] receives -- and records its
+  // own for the next step; the first step of a run reads T[2].
+  double pulse = 1.0;
+  if (c.pulsed) {  // wave-uniform
+    double tt;
+    if (kt.cur != nullptr) {
+      tt = kt.prev != nullptr ? kt.prev[2] + tdo : T[2];
+      if (blockIdx.x == 0 && threadIdx.x == 0) kt.cur[2] = tt;
+    } else {
+      tt = *TT;
+    }
+    pulse = pulse_factor(g, tt);
+  }
   step_dt_commit(T, kt, tdo);
   constexpr bool kDetects = METHOD == kExactLearned || METHOD == kExactDetect;
   typename std::conditional<STATS, StatsF32,
@@ -71,7 +89,7 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   int band, m0, m1;
   strip_rows(g.M, nbands, krows, xcd_tile(blockIdx.x, nwg), wave, band, m0, m1);
   tmin = strip_of_wave<METHOD, GEOM, WINDOW, SRC>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st,
-                                                  src);
+                                                  src, pulse);
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
@@ -81,7 +99,7 @@ template <int METHOD, int GEOM, bool WINDOW>
 int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const float2* s,
                       const float2* u, float2* s2, float2* u2, double* T, const double* T_do,
                       const KTime& kt, ast_stats_t* stats, int krows, hipStream_t stream,
-                      const SrcF* src) {
+                      const SrcF* src, const double* TT) {
   // krows > 0: strips of krows rows; krows = -S: S strips per band (strip_rows)
   const int nbands = (g.N + WOUT - 1) / WOUT;
   long long nwg;
@@ -102,7 +120,7 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
 #define AST_LAUNCH_SRC(STATS_, SRC_)                                                           \
   hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, STATS_, SRC_>), dim3(grid), dim3(NT), \
                      0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,    \
-                     (int)nwg, *src)
+                     (int)nwg, *src, TT)
       if (counting) {
         if constexpr (is_exact(METHOD)) {
           if (f32) AST_LAUNCH_SRC(true, kSrcF32);
@@ -121,11 +139,11 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
     if constexpr (is_exact(METHOD))
       hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true, kSrcNone>), dim3(grid), dim3(NT), 0,
                          stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
-                         (int)nwg, none);
+                         (int)nwg, none, TT);
   } else {
     hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false, kSrcNone>), dim3(grid), dim3(NT),
                        0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
-                       (int)nwg, none);
+                       (int)nwg, none, TT);
   }
   AST_LAUNCH_CHECK("ast_step_f32 launch");
   return AST_OK;
@@ -136,7 +154,7 @@ int step_launch(const StepArgs& a) {
   const bool axi = a.g.geometry == AST_AXISYMMETRIC;
 #define AST_LAUNCH(G, W)                                                                     \
   launch_f32<METHOD, G, W>(a.g, a.c, a.ex, a.s, a.u, a.s2, a.u2, a.T, a.T_do, a.kt, a.stats, \
-                           a.krows, a.stream, a.src)
+                           a.krows, a.stream, a.src, a.TT)
   if constexpr (METHOD == kExactLearned || METHOD == kExactDetect) {  // without a window only
     return axi ? AST_LAUNCH(AST_AXISYMMETRIC, false) : AST_LAUNCH(AST_PLANAR, false);
   } else {

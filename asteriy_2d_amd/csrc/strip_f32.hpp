@@ -76,7 +76,7 @@ AST_DEV void face_values(const Q& c, const Q& dl, const Q& dr, Q& plus, Q& minus
 // Cell (n, m) in FP32 with the ghost rules; corner/out-of-halo positions are
 // clamped into the grid (never read by an output cell, kept finite).
 AST_DEV Q load(const ast_grid_t& g, const float2* __restrict__ s, const float2* __restrict__ u,
-               int n, int m, bool cols_in) {
+               int n, int m, bool cols_in, double pulse) {
   float2 a, b;
   if (cols_in && m >= 0 && m < g.M) {
     const size_t k = (size_t)m * g.N + n;
@@ -92,7 +92,7 @@ AST_DEV Q load(const ast_grid_t& g, const float2* __restrict__ s, const float2* 
     a = s[k];
     b = u[k];
   } else {
-    fetch_cell(g, s, u, n, m, a, b);
+    fetch_cell(g, s, u, n, m, a, b, pulse);
   }
   return Q{a.x, a.y, b.x, b.y};
 }
@@ -118,6 +118,7 @@ struct Consts {
   float g, ig1, g1, gig1;  // gamma, 1/(gamma-1), gamma-1, gamma/(gamma-1)
   float fx0, iax2, iay2;   // frozen ellipse
   int frozen;
+  int pulsed;  // a side is AST_BC_DIRICHLET_PULSED
 };
 
 static inline Consts make_consts(const ast_grid_t& g) {
@@ -130,6 +131,8 @@ static inline Consts make_consts(const ast_grid_t& g) {
   c.fx0 = (float)g.frozen_x0;
   c.iax2 = c.frozen ? (float)(1.0 / (g.frozen_ax * g.frozen_ax)) : 0.f;
   c.iay2 = c.frozen ? (float)(1.0 / (g.frozen_ay * g.frozen_ay)) : 0.f;
+  c.pulsed = 0;
+  for (int k = 0; k < 4; k++) c.pulsed |= g.bc[k] == AST_BC_DIRICHLET_PULSED;
   return c;
 }
 
@@ -164,11 +167,9 @@ static inline bool strip_fits(const ast_grid_t& g, int krows) {
 // must hold a positive state.
 static inline int check_positive_bc(const ast_grid_t& g) {
   for (int k = 0; k < 4; k++) {
-    if (g.bc[k] == AST_BC_DIRICHLET && !(g.bc_state[k][0] > 0.0 && g.bc_state[k][1] > 0.0))
+    const bool dir = g.bc[k] == AST_BC_DIRICHLET || g.bc[k] == AST_BC_DIRICHLET_PULSED;
+    if (dir && !(g.bc_state[k][0] > 0.0 && g.bc_state[k][1] > 0.0))
       return ast_fail(AST_ERR_ARG, "FP32 step: Dirichlet side %d needs rho > 0 and p > 0", k);
-    if (g.bc[k] == AST_BC_DIRICHLET_PULSED)
-      return ast_fail(AST_ERR_ARG, "FP32 step: the pulsed inflow of side %d is served by the "
-                      "FP64 step only", k);
   }
   return AST_OK;
 }
@@ -346,6 +347,9 @@ struct StripArgs {
   bool gdir;
   float gsign, glo, ghi;
   float gq[4];
+  // density factor of pulsed Dirichlet ghosts (pulse_factor), read by the
+  // boundary variant, which serves every strip that holds such ghosts
+  double pulse;
   // fast edge strips (kFastE): the rules of the y-ghost rows, per side (0:
   // y lo, 1: y hi), all wave-uniform: a ghost row loads the mirror row
   // (yrefl) or the edge row, then vy -> med3(ysign * vy, ylo, yhi), or the
@@ -439,7 +443,7 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
       }
       return Q{p.x, p.y, q.x, q.y};
     } else {
-      return load(g, s, u, n, m, a.cols_in);
+      return load(g, s, u, n, m, a.cols_in, a.pulse);
     }
   };
   // last row a prefetch may name (the boundary variant's load() clamps any
@@ -710,7 +714,8 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
                              const float2* __restrict__ s, const float2* __restrict__ u,
                              float2* __restrict__ s2, float2* __restrict__ u2, double tdo,
                              int band, int m0, int m1, ST& st,
-                             const SrcF& src = SrcF{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f}) {
+                             const SrcF& src = SrcF{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f},
+                             double pulse = 1.0) {
   const int lane = threadIdx.x & 63;
   StripArgs a;
   a.src = src;
@@ -744,6 +749,7 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
   a.glo = (gbc == AST_BC_OUTFLOW_CLAMPED && side == AST_XHI) ? vc : -3.40282347e38f;
   a.ghi = (gbc == AST_BC_OUTFLOW_CLAMPED && side == AST_XLO) ? vc : 3.40282347e38f;
   for (int k = 0; k < 4; k++) a.gq[k] = (float)g.bc_state[side][k];
+  a.pulse = pulse;
   if (xfast) {  // the cell each lane loads: ghost rule, else clamped into the grid
     int src = a.n < 0 ? 0 : (a.n >= g.N ? g.N - 1 : a.n);
     const bool ghost = (a.n < 0 && a.n >= -2) || (a.n >= g.N && a.n <= g.N + 1);
@@ -762,12 +768,18 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
     const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 >= 4 && a.m1 - a.m0 <= 64 &&
                            (!is_exact(METHOD) || ((a.m1 - a.m0) & 3) == 0);
     const bool rows_in = a.m0 - 2 >= 0 && a.m1 + 1 < g.M;
-    if (fast_rows && rows_in && !cw.frozen) {
+    // strips holding ghosts of a pulsed side take the boundary variant
+    // (fetch_cell applies the pulse; the results are those of the fast
+    // variants, bit for bit)
+    const bool xpulse = c.pulsed && xfast && gbc == AST_BC_DIRICHLET_PULSED;
+    const bool ypulse = c.pulsed && (g.bc[AST_YLO] == AST_BC_DIRICHLET_PULSED ||
+                                     g.bc[AST_YHI] == AST_BC_DIRICHLET_PULSED);
+    if (fast_rows && rows_in && !cw.frozen && !xpulse) {
       if (a.cols_in)
         run_strip<METHOD, GEOM, WINDOW, kFast, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
       else
         run_strip<METHOD, GEOM, WINDOW, kFastX, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
-    } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0) {
+    } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0 && !xpulse && !ypulse) {
       // the y-ghost rules of fetch_cell as row rules (only the first or last
       // strips of a band hold ghost rows)
       for (int k = 0; k < 2; k++) {

@@ -213,8 +213,9 @@ int ast_mc_belong_point_f64(const ast_grid_t* g, size_t n, const double* xy, int
  * pairs each), writes s2,u2, reads *T_do, and lowers *T to the minimum
  * over all faces of krit*rad/max|wave speed| (capped at 1000,
  * kernel.cu:2530): a deterministic min, not the reference's racy store
- * (kernel.cu:3013-3017).  TT is accepted for signature parity and unused,
- * as in the reference's live body.
+ * (kernel.cu:3013-3017).  TT, the simulation time, is read only when a
+ * side is AST_BC_DIRICHLET_PULSED (add2's inflow perturbation); the
+ * reference's live body leaves it unused.
  * ast_step_f64 is the step in IEEE double: each face solved once and shared
  * by its two cells, slopes from neighbour differences; it agrees with the
  * reference's step (which solves each face from both cells with
