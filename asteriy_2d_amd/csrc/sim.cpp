@@ -57,15 +57,16 @@ struct ast_sim {
   bool collect;  // count the exact solver's work into stats (counting kernels)
   const ast_mlp_t* mlp;
   hipStream_t stream;
-  hipGraph_t graph[2];      // a captured run of kChunk steps, per learned mode
-  hipGraphExec_t exec[2];
-  bool graph_ok[2];
+  hipGraph_t graph[3];      // a captured run of kChunk steps, per run kind
+  hipGraphExec_t exec[3];
+  bool graph_ok[3];
   // learned mode (FP32 exact solver with a learned guess): each run of steps
-  // uses one solver kernel, the plain one (which flags strong two-shock
-  // faces) or the one with the learned start; the kernel that closes run k
-  // writes whether it met strong faces to hflags[k & 1] (pinned host
-  // memory), and run k + 2 takes that as its mode, after waiting for run k's
-  // completion event -- the host never waits for the run in flight
+  // uses one solver kernel -- the one with the learned start, the plain one
+  // that flags strong two-shock faces, or the plain one alone (kRunQuiet);
+  // the kernel that closes run k writes whether it met strong faces to
+  // hflags[k & 1] (pinned host memory), and run k + 2 takes that as its
+  // mode, after waiting for run k's completion event -- the host never
+  // waits for the run in flight
   uint32_t* hflags;         // pinned host, 2 words (fine-grained: kernel stores reach it)
   uint32_t* dflags;         // its device address
   hipEvent_t run_ev[2];     // completion of run k in run_ev[k & 1]
@@ -85,6 +86,12 @@ static size_t elem(const ast_sim_t* s) { return s->precision ? sizeof(float) : s
 
 // steps per captured graph (even: a replay ends on the plane it starts from)
 constexpr int kChunk = 20;
+// run kinds of the learned mode: plain solver that flags strong faces,
+// learned start (flags them too), plain solver without the flag (its runs
+// record "none"); a plain run flags only every kDetectEvery-th run, since
+// the flag costs the plain kernel ~4% and strong faces are rare
+constexpr int kRunPlain = 0, kRunLearned = 1, kRunQuiet = 2;
+constexpr int kDetectEvery = 4;
 
 static bool has_sources(const ast_sim_t* S) {
   return S->src.nn2 || S->src.nn3 || S->src32.xy || S->src32.e;
@@ -111,15 +118,17 @@ static ast::StepTime step_time(ast_sim_t* S, int j, int mode) {
   tm.cur = ast::slot_set(T, j);
   tm.rst = ast::slot_set(T, j + 1);
   tm.T = T;
-  tm.strong = mode >= 0 ? reinterpret_cast<uint32_t*>(T + ast::kGateWord) : nullptr;
-  tm.learned = mode > 0;
+  tm.strong = (mode == kRunPlain || mode == kRunLearned)
+                  ? reinterpret_cast<uint32_t*>(T + ast::kGateWord)
+                  : nullptr;
+  tm.learned = mode == kRunLearned;
   return tm;
 }
 
 // n steps from plane `from`.  FP32: step j lowers slot set j (mod 3), takes
 // its dt from set j-1 and advances the time; a fold of the last set closes
 // the run, so between runs the time block reads as after funk_time.  mode:
-// the learned mode of the run (-1: none).
+// the run kind in the learned mode (-1: not in it).
 static int launch_steps(ast_sim_t* S, int from, int n, int mode) {
   double* T = S->scal;
   for (int j = 0; j < n; j++) {
@@ -138,8 +147,9 @@ static int launch_steps(ast_sim_t* S, int from, int n, int mode) {
     } else {
       const ast::StepTime tm = step_time(S, j, mode);
       rc = ast_step_f32_rows(&S->g, (const float*)s, (const float*)u, (float*)s2, (float*)u2, T,
-                             T + 1, T + 2, S->method, S->mlp, S->collect ? S->stats : nullptr,
-                             S->krows, &tm, sources32(S), S->stream);
+                             T + 1, T + 2, S->method, mode == kRunQuiet ? nullptr : S->mlp,
+                             S->collect ? S->stats : nullptr, S->krows, &tm, sources32(S),
+                             S->stream);
     }
     if (rc != AST_OK) return rc;
   }
@@ -159,7 +169,7 @@ static int clear_slot_sets(ast_sim_t* S) {
 }
 
 static void drop_graph(ast_sim_t* S) {
-  for (int m = 0; m < 2; m++) {
+  for (int m = 0; m < 3; m++) {
     if (S->exec[m]) (void)hipGraphExecDestroy(S->exec[m]);
     if (S->graph[m]) (void)hipGraphDestroy(S->graph[m]);
     S->exec[m] = nullptr;
@@ -460,15 +470,17 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
   void* u2 = S->plane[2 * (1 - from) + 1];
   int best = S->precision == 1 ? S->krows : S->rows64;
   float best_t = 1e30f;
-  // the first step of a run (dt from T_do), in the kernel the runs take: with
-  // the learned start attached, the plain solver that detects strong faces
-  // unless the learned start is forced on
-  const ast::StepTime tm = step_time(S, 0, learned_mode(S) ? (S->mode_force == 1 ? 1 : 0) : -1);
+  // the first step of a run (dt from T_do), in the kernel most runs take:
+  // with the learned start attached, the plain solver without the strong-face
+  // flag unless the learned start is forced on
+  const int kind = learned_mode(S) ? (S->mode_force == 1 ? kRunLearned : kRunQuiet) : -1;
+  const ast::StepTime tm = step_time(S, 0, kind);
   auto step = [&](int kr) {
     return S->precision == 1
                ? ast_step_f32_rows(&S->g, (const float*)sp, (const float*)up, (float*)s2,
                                    (float*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
-                                   S->mlp, nullptr, kr, &tm, sources32(S), S->stream)
+                                   kind == kRunQuiet ? nullptr : S->mlp, nullptr, kr, &tm,
+                                   sources32(S), S->stream)
                : ast_step_f64_rows(&S->g, (const double*)sp, (const double*)up, (double*)s2,
                                    (double*)u2, S->scal, S->scal + 1, S->scal + 2, S->method,
                                    nullptr, kr, sources64(S), S->stream);
@@ -534,7 +546,7 @@ extern "C" int ast_sim_autotune(ast_sim_t* S, const int* candidates, int ncand, 
 }
 
 static int build_graph(ast_sim_t* S, int mode) {
-  const int m = mode > 0 ? 1 : 0;
+  const int m = mode < 0 ? 0 : mode;
   if (S->exec[m]) (void)hipGraphExecDestroy(S->exec[m]);
   if (S->graph[m]) (void)hipGraphDestroy(S->graph[m]);
   S->exec[m] = nullptr;
@@ -555,21 +567,24 @@ static int build_graph(ast_sim_t* S, int mode) {
   return AST_OK;
 }
 
-// The learned mode of the next run (learned_mode): its own initial value
-// for the first two runs, then the strong-face flag of the run two before,
-// read after that run's completion event (the run in between stays queued).
+// The kind of the next run (learned_mode): its own initial value for the
+// first two runs, then the learned start if the run two before met strong
+// faces (read after that run's completion event; the run in between stays
+// queued), else the plain solver, flagging strong faces every kDetectEvery
+// runs.  Forced plain runs do not flag.
 static int next_run_mode(ast_sim_t* S, int* mode) {
   if (S->mode_force >= 0) {
-    *mode = S->mode_force;
+    *mode = S->mode_force == 1 ? kRunLearned : kRunQuiet;
     return AST_OK;
   }
   const long long k = S->runs;
   if (k < 2) {
-    *mode = S->mode_init[k];
+    *mode = S->mode_init[k] ? kRunLearned : kRunPlain;
     return AST_OK;
   }
   CK(hipEventSynchronize(S->run_ev[k & 1]));
-  *mode = __atomic_load_n(&S->hflags[k & 1], __ATOMIC_ACQUIRE) ? 1 : 0;
+  const bool strong = __atomic_load_n(&S->hflags[k & 1], __ATOMIC_ACQUIRE) != 0u;
+  *mode = strong ? kRunLearned : (k % kDetectEvery == 0 ? kRunPlain : kRunQuiet);
   return AST_OK;
 }
 
@@ -580,7 +595,7 @@ static int one_run(ast_sim_t* S, int from, int n, bool graph) {
   int mode = -1, rc;
   if (lm && (rc = next_run_mode(S, &mode)) != AST_OK) return rc;
   if (graph) {
-    const int m = mode > 0 ? 1 : 0;
+    const int m = mode < 0 ? 0 : mode;
     if (!S->graph_ok[m] && (rc = build_graph(S, mode)) != AST_OK) return rc;
     CK(hipGraphLaunch(S->exec[m], S->stream));
   } else if ((rc = launch_steps(S, from, n, mode)) != AST_OK) {

@@ -392,11 +392,12 @@ int ast_sim_collect_stats(ast_sim_t* sim, int on);
 /* Learned guess (device ast_mlp_t, NULL = off) for the FP32 exact step.
  * The network costs the solver kernel registers (occupancy) whether or not a
  * face uses it, so the simulation runs it only where it can serve: each run
- * of 20 steps launches one solver kernel, the plain one (which flags strong
- * two-shock faces) or the one with the learned start, chosen from the flag
- * of the run two before (read after that run's completion, never waiting
- * for the run in flight); the first two runs after a create or load take
- * the plain solver (or the modes the checkpoint recorded). */
+ * of 20 steps launches one solver kernel, the plain one or the one with the
+ * learned start, chosen from the strong-two-shock-face flag of the run two
+ * before (read after that run's completion, never waiting for the run in
+ * flight); learned runs and every fourth plain run set that flag (the test
+ * costs the plain kernel ~4%); the first two runs after a create or load
+ * take the plain solver (or the modes the checkpoint recorded). */
 int ast_sim_set_mlp(ast_sim_t* sim, const ast_mlp_t* mlp);
 /* Neutral-H source planes for every later step (ast_step_f64_src,
  * ast_step_f32_src), copied from host memory: nn2 N*M triples, nn3 N*M

@@ -193,6 +193,51 @@ def test_learned_mode_follows_strong_faces(dev):
         assert sim.learned_modes() == (0, 0)  # the collision has passed
 
 
+def test_plain_runs_flag_strong_faces_every_fourth_run(dev):
+    """Plain runs flag strong faces only every fourth run (the flag costs the
+    plain kernel), forced plain runs never.  The collision's strong faces
+    last a few steps, so the colliding state is restored (set_state keeps
+    the run count) before runs 4 and 5: run 4 (4 % 4 = 0) flags them and
+    run 6 takes the learned start; run 5 meets them without flagging, so run
+    7 does not."""
+    from asteriy_2d_amd.sim import Simulation
+    print("plain runs flag every fourth run", flush=True)
+    g, s, u = _colliding()
+    model = learned.LearnedGuess.load(device=dev)
+    with Simulation(g, G.EXACT, "f32", s, u) as sim:
+        sim.set_mlp(model)
+        sim.set_learned_mode(0)
+        for _ in range(4):
+            sim.run(4)  # even: one run per call
+        sim.set_learned_mode(-1)
+        assert sim.learned_modes() == (0, 0)  # runs 4, 5: the forced runs flagged nothing
+        sim.set_state(s, u)
+        sim.run(4)                             # run 4 flags the collision
+        assert sim.learned_modes() == (0, 1)  # run 5 from run 3, run 6 from run 4
+        sim.set_state(s, u)
+        sim.run(4)                             # run 5 meets it without the flag
+        assert sim.learned_modes() == (1, 0)  # run 6 from run 4, run 7 from run 5
+
+
+def test_gated_learned_runs_step_like_the_plain_solver(dev):
+    """100 steps (five runs: two flagging, two quiet, one flagging) of the
+    wind workload with the network attached equal the plain exact solver's
+    bit for bit: no face is strong, and the flag changes no result."""
+    from asteriy_2d_amd.sim import Simulation
+    print("gated learned runs vs plain solver, 100 steps", flush=True)
+    p = problems.wind(256, 128)
+    model = learned.LearnedGuess.load(device=dev)
+    out = []
+    for mlp in (None, model):
+        with Simulation(p.grid, G.EXACT, "f32", p.s, p.u) as sim:
+            if mlp is not None:
+                sim.set_mlp(mlp)
+            sim.run(100)
+            out.append((sim.state(), sim.time()))
+    (sa, ta), (sb, tb) = out
+    assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1]) and ta == tb
+
+
 def test_learned_start_serves_strong_faces(dev):
     """Colliding streams with the learned start on for every run: its faces
     start newton from the network, the result agrees with the plain solver to
