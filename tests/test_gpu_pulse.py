@@ -13,13 +13,16 @@ perturbation, kernel.cu:2022-2025) in the FP64 step and the runtime:
 * the FP32 runtime (each step takes the time the previous one used plus its
   own dt): bit for bit the stand-alone FP32 step with funk_time, over graph
   and eager runs across t_lim; against the FP64 runtime;
+* the FP32 runtime with the pulse and neutral-H planes together: bit for bit
+  the stand-alone FP32 step with both;
 * the batch refuses the pulsed side (it has no simulation time)."""
 import numpy as np
 import pytest
 
 from asteriy_2d_amd import grid as G
 
-from cases import PARITY_REL, pulsed_case, pulsed_equivalent, step_mismatch
+from cases import (K_MOM_REF, PARITY_REL, neutral_sources, pulsed_case, pulsed_equivalent,
+                   step_mismatch)
 
 pytestmark = pytest.mark.gpu
 
@@ -251,3 +254,37 @@ def test_pulsed_runtime_fp32_vs_fp64(dev):
           f"FP32 vs FP64 {rel:.2e}", flush=True)
     assert l1 < 1e-6
     assert np.abs(e64).sum() > 0 and rel < 1e-3
+
+
+def test_pulsed_fp32_with_sources_runtime_equals_step(dev):
+    """The pulsed case with neutral-H planes: 26 steps of the FP32 runtime
+    (graph and eager runs, across t_lim) equal the stand-alone FP32 step with
+    both the time and the planes, and funk_time after each, bit for bit."""
+    import torch
+    from asteriy_2d_amd import solver
+    from asteriy_2d_amd.sim import Simulation
+    print("FP32 pulsed runtime with sources vs step", flush=True)
+    g, s, u = _wide_case(True)
+    nn2, nn3 = neutral_sources(g.M, g.N, seed=11, scale=2e-3, scale3=1e-4)
+    with Simulation(g, G.HLLC, "f32", s, u) as sim:
+        sim.set_sources(nn2, nn3, K_MOM_REF, 0.02)
+        sim.run(3, graph=False)
+        sim.run(23)
+        sa, ua = sim.state()
+        ta = sim.time()
+    src = solver.NeutralSources(torch.from_numpy(nn2).to(dev), torch.from_numpy(nn3).to(dev),
+                                K_MOM_REF, 0.02)
+    S = torch.from_numpy(s).to(dev, torch.float32)
+    U = torch.from_numpy(u).to(dev, torch.float32)
+    S2, U2 = torch.empty_like(S), torch.empty_like(U)
+    T = torch.tensor([1e7], dtype=torch.float64, device=dev)
+    Td = torch.tensor([1e-9], dtype=torch.float64, device=dev)
+    TT = torch.zeros(1, dtype=torch.float64, device=dev)
+    for _ in range(26):
+        solver.Kernel_TVD(g, S, U, S2, U2, T, Td, TT, G.HLLC, sources=src)
+        solver.funk_time(T, Td, TT)
+        S, S2, U, U2 = S2, S, U2, U
+    torch.cuda.synchronize()
+    assert np.array_equal(sa, S.double().cpu().numpy())
+    assert np.array_equal(ua, U.double().cpu().numpy())
+    assert ta["TT"] == float(TT.item()) and ta["i"] == 26
