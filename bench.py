@@ -47,10 +47,11 @@ def log(*a):
 
 def _timed(runner, steps: int, stream, graph: bool = False):
     """`steps` steps between HIP events on the runner's stream.  The
-    simulations launch their step kernels directly on the stream (graph
-    replay gives the same bits and measured 1.7% slower per step at
-    2048x1024: each step kernel is long enough that the host keeps the
-    queue full); the batch replays its captured launch (graph=True)."""
+    simulations launch their step kernels directly on the stream: one run of
+    `steps` steps closed by one fold kernel (graph replay gives the same bits
+    and closes every 20-step run with a fold: 1.5-1.7% slower per step at
+    2048x1024; each step kernel is long enough that the host keeps the queue
+    full either way); the batch replays its captured launch (graph=True)."""
     import torch
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
