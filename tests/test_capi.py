@@ -114,6 +114,31 @@ def test_step_rejects_bad_arguments_without_gpu(lib):
     assert rc == 1
 
 
+def test_pulsed_side_needs_the_time_without_gpu(lib):
+    """A pulsed Dirichlet side: the FP32 and FP64 steps refuse a null TT, and
+    the batch (no per-launch time) refuses the side, before any device
+    work."""
+    from asteriy_2d_amd import grid as G
+    from asteriy_2d_amd import native
+    L = native.load()
+    g = G.Grid.from_extent(64, 32, 0.0, 1.0, 1.0, bc=(G.BC_DIRICHLET_PULSED, 0, 0, 0),
+                           bc_state=[[1.0, 1.0, 0.5, 0.0], [0.0] * 4, [0.0] * 4, [0.0] * 4],
+                           bc_pulse=(0.01, 6.283185307179586, 0.5, 3.0))
+    buf = (ctypes.c_double * 16)()
+    a = [ctypes.c_void_p(ctypes.addressof(buf) + 8 * k) for k in range(6)]
+    rc = L.ast_step_f32(native.grid_ref(g), a[0], a[1], a[2], a[3], a[4], a[5], None, 1, None,
+                        None, None)
+    assert rc == 1 and b"TT" in L.ast_last_error()
+    rc = L.ast_step_f64(native.grid_ref(g), a[0], a[1], a[2], a[3], a[4], a[5], None, 1, None,
+                        None)
+    assert rc == 1 and b"TT" in L.ast_last_error()
+    cg = g.to_c()
+    grids = (type(cg) * 1)(cg)
+    h = ctypes.c_void_p()
+    rc = L.ast_batch_create(1, grids, 1, ctypes.byref(h))
+    assert rc == 1 and b"pulsed" in L.ast_last_error()
+
+
 def test_driver_runs_without_gpu_as_usage_only():
     import subprocess
     from asteriy_2d_amd import build
