@@ -5,15 +5,20 @@
 // FP64 face solvers of flux64.hpp -- the code the FP64 HIP kernels run,
 // compiled for the host -- over random, NaN, vacuum and extreme faces in
 // every pattern, with and without a learned newton start and a passive
-// scalar.  Exit status 0 and "host sancheck ok" mean every call returned.
+// scalar; and the Monte-Carlo helpers of asteriy_mc.hpp over random
+// particles, rays and rate pairs (both branches of the rates, rays along
+// the axes and through the axis).  Exit status 0 and "host sancheck ok" mean
+// every call returned.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <vector>
 
 #include "../asteriy_2d_amd/csrc/flux64.hpp"
 #include "../include/asteriy.h"
+#include "../include/asteriy_mc.hpp"
 
 int ast_fail(int code, const char* fmt, ...);
 
@@ -104,9 +109,59 @@ static void check_faces() {
   }
 }
 
+static void check_mc() {
+  namespace mc = asteriy::mc;
+  ast_grid_t g;
+  memset(&g, 0, sizeof g);
+  g.N = 200;
+  g.M = 40;
+  g.x_min = -10.0;
+  g.dx = 0.1;
+  g.dy = 0.025;
+  g.y_min = 0.0125;
+  const mc::McGrid G = mc::make_grid(g);
+  int bad = 0;
+  for (int k = 0; k < 20000; k++) {
+    // rates: both sides of the u < 0.001 switch, fused equal to separate
+    const double cp = 0.05 + 3.0 * urand();
+    const double u = (k & 3) == 0 ? 0.0 : ((k & 3) == 1 ? 2e-3 * urand() : 10.0 * urand());
+    double v1, v2, v3;
+    mc::velosities(u, cp, v1, v2, v3);
+    if (!(v1 == mc::velosity_1(u, cp)) || !(v2 == mc::velosity_2(u, cp)) ||
+        !(v3 == mc::velosity_3(u, cp)))
+      bad++;
+    if (!isfinite(v1) || !isfinite(v2) || !isfinite(v3)) bad++;
+    // a particle inside cell (n, m), a ray in any direction (some along an
+    // axis, some through the axis r = 0), its exit and the cell of a point
+    const int n = (int)(urand() * g.N), m = (int)(urand() * g.M);
+    const double x0 = g.x_min + n * g.dx, y0 = g.y_min + m * g.dy;
+    const double x = x0 + (urand() - 0.5) * 0.9 * g.dx;
+    const double r = y0 + (urand() - 0.5) * 0.9 * g.dy;
+    const double ph = 6.283185307179586 * urand();
+    const double y = r * cos(ph), z = r * sin(ph);
+    double Vx = urand() - 0.5, Vy = urand() - 0.5, Vz = urand() - 0.5;
+    if (k % 7 == 0) Vy = Vz = 0.0;
+    if (k % 11 == 0) Vx = 0.0;
+    if (k % 13 == 0) { Vy = -y; Vz = -z; }
+    int mode = -1;
+    double t = -1.0;
+    (void)mc::peresechenie(G, x0, y0, x, y, z, Vx, Vy, Vz, mode, t);
+    if (mode < 0 || mode > 4) bad++;
+    int bn = -1, bm = -1;
+    mc::belong_point(G, x, r, bn, bm);
+    if (bn < n - 1 || bn > n + 1 || bm < m - 1 || bm > m + 1) bad++;
+    if (mc::minplus(-1.0, -2.0) != -1.0) bad++;
+  }
+  if (bad) {
+    printf("mc helpers: %d bad results\n", bad);
+    exit(1);
+  }
+}
+
 int main() {
   check_api();
   check_faces();
+  check_mc();
   printf("host sancheck ok\n");
   return 0;
 }
