@@ -12,7 +12,9 @@ behind the FP64 step (ast_step_f64_src) and the runtime (ast_sim_set_sources):
   planes, zero planes bit for bit the plain FP32 step, and the FP32 runtime
   against the FP64 runtime with the planes;
 * the FP32 runtime, which keeps the planes' FP32 roundings, against the
-  stand-alone FP32 step reading the FP64 planes: bit for bit."""
+  stand-alone FP32 step reading the FP64 planes: bit for bit;
+* at the headline size (2048 x 1024), 40 steps of the FP32 runtime with the
+  planes against the FP64 runtime with them."""
 import numpy as np
 import pytest
 
@@ -290,3 +292,35 @@ def test_sources_fp32_runtime_equals_step(dev, method, tile, stats):
     assert np.array_equal(sa, S.double().cpu().numpy())
     assert np.array_equal(ua, U.double().cpu().numpy())
     assert ta["TT"] == float(TT.item()) and ta["i"] == steps
+
+
+def test_sources_fp32_full_size(dev):
+    """The wind workload at 2048 x 1024 with random planes (kmom = n_H/Kn,
+    ken = 0.02): 40 steps of the FP32 runtime against the FP64 runtime,
+    relative L1 of rho and p below 1e-6, and the planes' own effect (against
+    the runs without them) the same in both precisions to 1e-3 of its size."""
+    from asteriy_2d_amd import problems
+    from asteriy_2d_amd.sim import Simulation
+    print("FP32 sources at 2048x1024", flush=True)
+    p = problems.wind(2048, 1024, method=G.HLLC)
+    g = p.grid
+    rng = np.random.default_rng(5)
+    nn2 = rng.uniform(-1e-3, 1e-3, (g.M, g.N, 3))
+    nn3 = rng.uniform(-1e-4, 1e-4, (g.M, g.N))
+    res = {}
+    for prec in ("f32", "f64"):
+        for planes in (True, False):
+            with Simulation(g, G.HLLC, prec, p.s, p.u) as sim:
+                if planes:
+                    sim.set_sources(nn2, nn3, K_MOM_REF, 0.02)
+                sim.run(40)
+                res[prec, planes] = sim.state()[0]
+    for c in (0, 1):
+        a, b = res["f32", True][..., c], res["f64", True][..., c]
+        l1 = np.abs(a - b).sum() / np.abs(b).sum()
+        e32 = res["f32", True][..., c] - res["f32", False][..., c]
+        e64 = res["f64", True][..., c] - res["f64", False][..., c]
+        rel = np.abs(e32 - e64).sum() / np.abs(e64).sum()
+        print(f"  component {c}: relative L1 {l1:.2e}; planes' effect {np.abs(e64).sum():.3e}, "
+              f"FP32 vs FP64 {rel:.2e}", flush=True)
+        assert l1 < 1e-6 and np.abs(e64).sum() > 0 and rel < 1e-3, (c, l1, rel)
