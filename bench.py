@@ -655,12 +655,16 @@ def main(argv=None):
         k32s = _kernel_ms(g, "f32", HLLC, torch.cuda.current_stream(), krows=rows, reps=100,
                           sources=solver.NeutralSources(nn2, nn3, ken=0.02))
         del nn2, nn3, h2, h3
+        # and with add2's pulsed inflow (the pulsed band's strips take the
+        # boundary variant), through the stand-alone entry point
+        k32p = _kernel_ms(gp, "f32", HLLC, torch.cuda.current_stream(), krows=rows, reps=100)
         extra["fp32_step_variants"] = {
             "plain_us": round(kms32 * 1e3, 2),
             "neutral_sources_runtime_us": round(k32r * 1e3, 2),
             "neutral_sources_runtime_hbm_frac": round(44.0 * cells / (k32r * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "neutral_sources_fp64_planes_us": round(k32s * 1e3, 2),
-            "neutral_sources_fp64_planes_hbm_frac": round(64.0 * cells / (k32s * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "neutral_sources_fp64_planes_hbm_frac": round(64.0 * cells / (k32s * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pulsed_inflow_us": round(k32p * 1e3, 2)}
         log(f"FP32 step variants: {extra['fp32_step_variants']}")
         sr = p.s.astype(np.float32).astype(np.float64)
         ur = p.u.astype(np.float32).astype(np.float64)
