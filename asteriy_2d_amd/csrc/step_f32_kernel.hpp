@@ -48,8 +48,10 @@ extern template int step_launch<kExactDetect>(const StepArgs&);
 
 // STATS: the variant that counts the exact solver's work into *stats; SRC:
 // the variants with add_MK's neutral-hydrogen source planes (kSrcF64, the
-// reference's planes; kSrcF32, FP32 copies of them)
-template <int METHOD, int GEOM, bool WINDOW, bool STATS, int SRC>
+// reference's planes; kSrcF32, FP32 copies of them); PULSE: the variant for
+// grids with a pulsed Dirichlet side (HLL family), whose fast edge strips
+// apply the pulse -- the kernels without it keep their code unchanged
+template <int METHOD, int GEOM, bool WINDOW, bool STATS, int SRC, bool PULSE>
 __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactCtx ex,
                                                   const float2* __restrict__ s,
                                                   const float2* __restrict__ u,
@@ -88,8 +90,8 @@ __global__ __launch_bounds__(NT) void step_kernel(ast_grid_t g, Consts c, ExactC
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int band, m0, m1;
   strip_rows(g.M, nbands, krows, xcd_tile(blockIdx.x, nwg), wave, band, m0, m1);
-  tmin = strip_of_wave<METHOD, GEOM, WINDOW, SRC>(g, c, ex, s, u, s2, u2, tdo, band, m0, m1, st,
-                                                  src, pulse);
+  tmin = strip_of_wave<METHOD, GEOM, WINDOW, SRC, PULSE>(g, c, ex, s, u, s2, u2, tdo, band, m0,
+                                                         m1, st, src, pulse);
   if (ex.strong != nullptr && st.any_strong() && (threadIdx.x & 63) == 0) atomicOr(ex.strong, 1u);
   if constexpr (STATS) block_stats_f32_to<NT>(st, redi, stats);
   block_min_uniform<NT>(tmin, red, kt.cur ? kt.cur : T, kt.cur ? kt.nslots : 0);
@@ -113,38 +115,48 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
   const unsigned grid = (unsigned)nwg;
   const SrcF none{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f};
   constexpr bool kSrc = METHOD != kExactLearned && METHOD != kExactDetect;
+  // pulsed sides: the PULSE kernels of the HLL family (the exact solver's
+  // kernels serve them on the boundary strip path)
+  constexpr bool kPulseKernels = !is_exact(METHOD);
+  const bool pulsed = kPulseKernels && c.pulsed;
+#define AST_LAUNCH_K(STATS_, SRC_, PULSE_, SRCV)                                                    \
+  hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, STATS_, SRC_, PULSE_>), dim3(grid),        \
+                     dim3(NT), 0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows, \
+                     (int)nwg, SRCV, TT)
+#define AST_LAUNCH_P(STATS_, SRC_, SRCV)        \
+  do {                                          \
+    if constexpr (kPulseKernels) {              \
+      if (pulsed) {                             \
+        AST_LAUNCH_K(STATS_, SRC_, true, SRCV); \
+        break;                                  \
+      }                                         \
+    }                                           \
+    AST_LAUNCH_K(STATS_, SRC_, false, SRCV);    \
+  } while (0)
   if (src != nullptr) {
     if constexpr (kSrc) {
       const bool f32 = src->xy != nullptr || src->e != nullptr;
       const bool counting = is_exact(METHOD) && stats != nullptr;
-#define AST_LAUNCH_SRC(STATS_, SRC_)                                                           \
-  hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, STATS_, SRC_>), dim3(grid), dim3(NT), \
-                     0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,    \
-                     (int)nwg, *src, TT)
       if (counting) {
         if constexpr (is_exact(METHOD)) {
-          if (f32) AST_LAUNCH_SRC(true, kSrcF32);
-          else AST_LAUNCH_SRC(true, kSrcF64);
+          if (f32) AST_LAUNCH_K(true, kSrcF32, false, *src);
+          else AST_LAUNCH_K(true, kSrcF64, false, *src);
         }
       } else if (f32) {
-        AST_LAUNCH_SRC(false, kSrcF32);
+        AST_LAUNCH_P(false, kSrcF32, *src);
       } else {
-        AST_LAUNCH_SRC(false, kSrcF64);
+        AST_LAUNCH_P(false, kSrcF64, *src);
       }
-#undef AST_LAUNCH_SRC
     } else {
       return ast_fail(AST_ERR_ARG, "ast_step_f32: source planes with the learned start");
     }
   } else if (is_exact(METHOD) && stats != nullptr) {
-    if constexpr (is_exact(METHOD))
-      hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, true, kSrcNone>), dim3(grid), dim3(NT), 0,
-                         stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
-                         (int)nwg, none, TT);
+    if constexpr (is_exact(METHOD)) AST_LAUNCH_K(true, kSrcNone, false, none);
   } else {
-    hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, false, kSrcNone>), dim3(grid), dim3(NT),
-                       0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows,
-                       (int)nwg, none, TT);
+    AST_LAUNCH_P(false, kSrcNone, none);
   }
+#undef AST_LAUNCH_P
+#undef AST_LAUNCH_K
   AST_LAUNCH_CHECK("ast_step_f32 launch");
   return AST_OK;
 }

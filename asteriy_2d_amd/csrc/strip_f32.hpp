@@ -396,11 +396,35 @@ struct Row {
 // the whole row, the frozen tests of kEdge.
 constexpr int kEdge = 0, kFast = 1, kFastX = 2, kFastE = 3;
 
-template <int METHOD, int GEOM, bool WINDOW, int PATH, int SRC, class ST>
+// The ghost densities of pulsed Dirichlet sides in the fast edge variants
+// (kernels compiled for grids with a pulsed side), applied to each loaded
+// row: rows m < prows take the pulsed density on this lane if it is a ghost
+// column of a pulsed x side (gl), and in the y ghost rows of a pulsed y side
+// (plo, phi > 0; -1 where the side is not pulsed).  Each select is between
+// a member and a value (two members would fold into a load through a
+// selected address, which keeps the struct in scratch memory).  NoPulse, the
+// other kernels' patch, leaves the rows as loaded.
+struct NoPulse {
+  AST_DEV Q operator()(const Q& q, int) const { return q; }
+};
+struct PulsePatch {
+  bool gl;
+  int prows, M;
+  float px, plo, phi;
+  AST_DEV Q operator()(Q q, int m) const {
+    const bool below = m < prows;
+    q.r = (gl && below) ? px : q.r;
+    q.r = (m < 0 && below && plo > 0.f) ? plo : q.r;
+    q.r = (m >= M && below && phi > 0.f) ? phi : q.r;
+    return q;
+  }
+};
+
+template <int METHOD, int GEOM, bool WINDOW, int PATH, int SRC, class ST, class PF = NoPulse>
 AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, const ExactCtx& ex,
                        const float2* __restrict__ s, const float2* __restrict__ u,
                        float2* __restrict__ s2, float2* __restrict__ u2, const StripArgs& a,
-                       float& UUx, float& UUy, ST& st) {
+                       float& UUx, float& UUy, ST& st, const PF& pf = PF{}) {
   constexpr bool FAST = PATH != kEdge;
   const int n = a.n, m0 = a.m0, m1 = a.m1, N = g.N, M = g.M;
   const unsigned rb = a.row_bytes;
@@ -461,10 +485,10 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
   FaceF G[2];
   bool fr[2] = {false, false};
   {
-    const Q qa = LD(m0 - 2), qb = LD(m0 - 1);
-    w[0] = LD(m0);
-    w[1] = LD(m0 + 1);
-    w[2] = LD(m0 + 2);
+    const Q qa = pf(LD(m0 - 2), m0 - 2), qb = pf(LD(m0 - 1), m0 - 1);
+    w[0] = pf(LD(m0), m0);
+    w[1] = pf(LD(m0 + 1), m0 + 1);
+    w[2] = pf(LD(m0 + 2), m0 + 2);
     const Q d0 = sub(qb, qa), d1 = sub(w[0], qb);
     dd[0] = sub(w[1], w[0]);
     Q nvm1, svm1, sv0;
@@ -506,7 +530,7 @@ AST_DEV void run_strip(const ast_grid_t& g, const Consts& c, const Consts& cw, c
     const Q& q0 = w[R::w0];
     const Q& q1 = w[R::w1];
     const Q& q2 = w[R::w2];
-    w[R::w3] = LD(min(m + 3, mlast));
+    w[R::w3] = pf(LD(min(m + 3, mlast)), min(m + 3, mlast));
     SrcV<SRC> snext{0, 0, 0};
     // (the last row of the strip re-reads its own values, which are cached,
     // rather than the next strip's)
@@ -709,7 +733,7 @@ AST_DEV void strip_rows(int M, int nbands, int tile, int wg, int wave, int& band
 // The strip of one wave given its band and rows (strip_rows): fills
 // StripArgs and runs the fast or the boundary variant.  Returns the wave's
 // dt candidate (wave-uniform).
-template <int METHOD, int GEOM, bool WINDOW, int SRC = kSrcNone, class ST>
+template <int METHOD, int GEOM, bool WINDOW, int SRC = kSrcNone, bool PULSE = false, class ST>
 AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCtx& ex,
                              const float2* __restrict__ s, const float2* __restrict__ u,
                              float2* __restrict__ s2, float2* __restrict__ u2, double tdo,
@@ -768,34 +792,77 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
     const bool fast_rows = (a.cols_in || xfast) && a.m1 - a.m0 >= 4 && a.m1 - a.m0 <= 64 &&
                            (!is_exact(METHOD) || ((a.m1 - a.m0) & 3) == 0);
     const bool rows_in = a.m0 - 2 >= 0 && a.m1 + 1 < g.M;
-    // strips holding ghosts of a pulsed side take the boundary variant
-    // (fetch_cell applies the pulse; the results are those of the fast
-    // variants, bit for bit)
-    const bool xpulse = c.pulsed && xfast && gbc == AST_BC_DIRICHLET_PULSED;
-    const bool ypulse = c.pulsed && (g.bc[AST_YLO] == AST_BC_DIRICHLET_PULSED ||
-                                     g.bc[AST_YHI] == AST_BC_DIRICHLET_PULSED);
-    if (fast_rows && rows_in && !cw.frozen && !xpulse) {
-      if (a.cols_in)
-        run_strip<METHOD, GEOM, WINDOW, kFast, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
-      else
-        run_strip<METHOD, GEOM, WINDOW, kFastX, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
-    } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0 && !xpulse && !ypulse) {
-      // the y-ghost rules of fetch_cell as row rules (only the first or last
-      // strips of a band hold ghost rows)
-      for (int k = 0; k < 2; k++) {
-        const int ys = k ? AST_YHI : AST_YLO;
-        const int ybc = g.bc[ys];
-        const float yc = (float)g.bc_vclamp[ys];
-        a.yrefl[k] = ybc == AST_BC_REFLECT;
-        a.ydir[k] = ybc == AST_BC_DIRICHLET;
-        a.ysign[k] = ybc == AST_BC_REFLECT ? -1.f : 1.f;
-        a.ylo[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 1) ? yc : -3.40282347e38f;
-        a.yhi[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 0) ? yc : 3.40282347e38f;
-        for (int j = 0; j < 4; j++) a.yq[k][j] = (float)g.bc_state[ys][j];
+    if constexpr (PULSE) {
+      // a grid with a pulsed side: the strips that hold its ghosts run the
+      // fast variants with the pulsed densities applied to each loaded row
+      PulsePatch pp;
+      pp.prows = -2;  // the first row that is not pulsed_row (monotone in m)
+      for (int hi = g.M + 2; pp.prows < hi;) {
+        const int mid = pp.prows + (hi - pp.prows) / 2;
+        if (pulsed_row(g, mid)) pp.prows = mid + 1;
+        else hi = mid;
       }
-      run_strip<METHOD, GEOM, WINDOW, kFastE, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      pp.M = g.M;
+      pp.px = pp.plo = pp.phi = -1.f;
+      if (gbc == AST_BC_DIRICHLET_PULSED)
+        pp.px = (float)((xlo_band ? g.bc_state[AST_XLO][0] : g.bc_state[AST_XHI][0]) * pulse);
+      if (g.bc[AST_YLO] == AST_BC_DIRICHLET_PULSED) pp.plo = (float)(g.bc_state[AST_YLO][0] * pulse);
+      if (g.bc[AST_YHI] == AST_BC_DIRICHLET_PULSED) pp.phi = (float)(g.bc_state[AST_YHI][0] * pulse);
+      pp.gl = xfast && pp.px > 0.f && ((a.gmask >> lane) & 1ull);
+      a.gdir = a.gdir || gbc == AST_BC_DIRICHLET_PULSED;
+      if (fast_rows && rows_in && !cw.frozen) {
+        if (a.cols_in)
+          run_strip<METHOD, GEOM, WINDOW, kFast, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+        else
+          run_strip<METHOD, GEOM, WINDOW, kFastX, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st,
+                                                       pp);
+      } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0) {
+        for (int k = 0; k < 2; k++) {
+          const int ys = k ? AST_YHI : AST_YLO;
+          const int ybc = g.bc[ys];
+          const float yc = (float)g.bc_vclamp[ys];
+          a.yrefl[k] = ybc == AST_BC_REFLECT;
+          a.ydir[k] = ybc == AST_BC_DIRICHLET || ybc == AST_BC_DIRICHLET_PULSED;
+          a.ysign[k] = ybc == AST_BC_REFLECT ? -1.f : 1.f;
+          a.ylo[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 1) ? yc : -3.40282347e38f;
+          a.yhi[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 0) ? yc : 3.40282347e38f;
+          for (int j = 0; j < 4; j++) a.yq[k][j] = (float)g.bc_state[ys][j];
+        }
+        run_strip<METHOD, GEOM, WINDOW, kFastE, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st,
+                                                     pp);
+      } else {
+        run_strip<METHOD, GEOM, WINDOW, kEdge, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      }
     } else {
-      run_strip<METHOD, GEOM, WINDOW, kEdge, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      // strips holding ghosts of a pulsed side take the boundary variant
+      // (fetch_cell applies the pulse; the results are those of the fast
+      // variants, bit for bit)
+      const bool xpulse = c.pulsed && xfast && gbc == AST_BC_DIRICHLET_PULSED;
+      const bool ypulse = c.pulsed && (g.bc[AST_YLO] == AST_BC_DIRICHLET_PULSED ||
+                                       g.bc[AST_YHI] == AST_BC_DIRICHLET_PULSED);
+      if (fast_rows && rows_in && !cw.frozen && !xpulse) {
+        if (a.cols_in)
+          run_strip<METHOD, GEOM, WINDOW, kFast, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+        else
+          run_strip<METHOD, GEOM, WINDOW, kFastX, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      } else if (fast_rows && ((a.m1 - a.m0) & 3) == 0 && !xpulse && !ypulse) {
+        // the y-ghost rules of fetch_cell as row rules (only the first or last
+        // strips of a band hold ghost rows)
+        for (int k = 0; k < 2; k++) {
+          const int ys = k ? AST_YHI : AST_YLO;
+          const int ybc = g.bc[ys];
+          const float yc = (float)g.bc_vclamp[ys];
+          a.yrefl[k] = ybc == AST_BC_REFLECT;
+          a.ydir[k] = ybc == AST_BC_DIRICHLET;
+          a.ysign[k] = ybc == AST_BC_REFLECT ? -1.f : 1.f;
+          a.ylo[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 1) ? yc : -3.40282347e38f;
+          a.yhi[k] = (ybc == AST_BC_OUTFLOW_CLAMPED && k == 0) ? yc : 3.40282347e38f;
+          for (int j = 0; j < 4; j++) a.yq[k][j] = (float)g.bc_state[ys][j];
+        }
+        run_strip<METHOD, GEOM, WINDOW, kFastE, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      } else {
+        run_strip<METHOD, GEOM, WINDOW, kEdge, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
+      }
     }
   }
   // only output lanes vote (halo lanes may hold garbage speeds)

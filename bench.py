@@ -655,8 +655,8 @@ def main(argv=None):
         k32s = _kernel_ms(g, "f32", HLLC, torch.cuda.current_stream(), krows=rows, reps=100,
                           sources=solver.NeutralSources(nn2, nn3, ken=0.02))
         del nn2, nn3, h2, h3
-        # and with add2's pulsed inflow (the pulsed band's strips take the
-        # boundary variant), through the stand-alone entry point
+        # and with add2's pulsed inflow (the HLL family's kernels for pulsed
+        # grids), through the stand-alone entry point
         k32p = _kernel_ms(gp, "f32", HLLC, torch.cuda.current_stream(), krows=rows, reps=100)
         extra["fp32_step_variants"] = {
             "plain_us": round(kms32 * 1e3, 2),
