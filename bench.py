@@ -461,7 +461,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configs)")
-    ap.add_argument("--profile", choices=["f32", "f32src", "f32src64", "f64", "f64src",
+    ap.add_argument("--profile", choices=["f32", "f32src", "f32src64", "f32pulse", "f64", "f64src",
                                           "f64pulse", "exact32",
                                           "exact32l", "exact64", "batch", "flux", "mhd",
                                           "mc"],
@@ -469,8 +469,8 @@ def main(argv=None):
                     help="only run --steps steps of the wind workload (for rocprofv3): the HLLC "
                          "step in FP32 / FP64 from the initial state (f32src / f64src: with "
                          "neutral-H planes in the runtime, f32src64: the FP32 step's C entry "
-                         "point with the FP64 planes, f64pulse: with add2's pulsed inflow on "
-                         "the Dirichlet side), "
+                         "point with the FP64 planes, f32pulse / f64pulse: with add2's pulsed "
+                         "inflow on the Dirichlet side), "
                          "or the exact-solver step (FP32, FP32 with the learned guess, FP64); "
                          "or --steps steps of the 64-instance sweep (batch); or the "
                          "Riemann-flux microbenchmark (flux), or its MHD fluxes alone (mhd), or "
@@ -564,11 +564,12 @@ def main(argv=None):
                        reps=K, sources=solver.NeutralSources(nn2, nn3, ken=0.02))
             log(f"step_{a.profile} HLLC {gp.N}x{gp.M}: {K} steps")
             return 0
-        if a.profile == "f64pulse":
+        if a.profile in ("f32pulse", "f64pulse"):
             from asteriy_2d_amd import BC_DIRICHLET_PULSED, PULSE_ADD2
             gp = gp.replace(bc=(gp.bc[0], BC_DIRICHLET_PULSED, gp.bc[2], gp.bc[3]),
                             bc_pulse=PULSE_ADD2)
-        sim = Simulation(gp, HLLC, "f32" if a.profile in ("f32", "f32src") else "f64", p.s, p.u)
+        sim = Simulation(gp, HLLC, "f32" if a.profile in ("f32", "f32src", "f32pulse") else "f64",
+                         p.s, p.u)
         if a.profile in ("f32src", "f64src"):
             rng = np.random.default_rng(9)
             sim.set_sources(rng.uniform(-1e-3, 1e-3, (gp.M, gp.N, 3)),
