@@ -348,7 +348,8 @@ struct StripArgs {
   float gsign, glo, ghi;
   float gq[4];
   // density factor of pulsed Dirichlet ghosts (pulse_factor), read by the
-  // boundary variant, which serves every strip that holds such ghosts
+  // boundary variant (the fast variants of the PULSE kernels take the pulsed
+  // densities from a PulsePatch)
   double pulse;
   // fast edge strips (kFastE): the rules of the y-ghost rows, per side (0:
   // y lo, 1: y hi), all wave-uniform: a ghost row loads the mirror row
@@ -834,9 +835,10 @@ AST_DEV double strip_of_wave(const ast_grid_t& g, const Consts& c, const ExactCt
         run_strip<METHOD, GEOM, WINDOW, kEdge, SRC>(g, c, cw, ex, s, u, s2, u2, a, UUx, UUy, st);
       }
     } else {
-      // strips holding ghosts of a pulsed side take the boundary variant
-      // (fetch_cell applies the pulse; the results are those of the fast
-      // variants, bit for bit)
+      // kernels without the PULSE variants (the exact solver's): strips
+      // holding ghosts of a pulsed side take the boundary variant (fetch_cell
+      // applies the pulse; the results are those of the fast variants, bit
+      // for bit)
       const bool xpulse = c.pulsed && xfast && gbc == AST_BC_DIRICHLET_PULSED;
       const bool ypulse = c.pulsed && (g.bc[AST_YLO] == AST_BC_DIRICHLET_PULSED ||
                                        g.bc[AST_YHI] == AST_BC_DIRICHLET_PULSED);
