@@ -115,17 +115,19 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
   const unsigned grid = (unsigned)nwg;
   const SrcF none{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f};
   constexpr bool kSrc = METHOD != kExactLearned && METHOD != kExactDetect;
-  // pulsed sides: the PULSE kernels of the HLL family (the exact solver's
-  // kernels serve them on the boundary strip path)
-  constexpr bool kPulseKernels = !is_exact(METHOD);
+  // pulsed sides: the PULSE kernels of the HLL family, and of the plain
+  // exact solver without counters or source planes (the other exact-solver
+  // kernels serve pulsed sides on the boundary strip path)
+  constexpr bool kPulseKernels = !is_exact(METHOD) || METHOD == AST_EXACT;
+  constexpr bool kPulseSrcKernels = !is_exact(METHOD);
   const bool pulsed = kPulseKernels && c.pulsed;
 #define AST_LAUNCH_K(STATS_, SRC_, PULSE_, SRCV)                                                    \
   hipLaunchKernelGGL((step_kernel<METHOD, GEOM, WINDOW, STATS_, SRC_, PULSE_>), dim3(grid),        \
                      dim3(NT), 0, stream, g, c, ex, s, u, s2, u2, T, T_do, kt, stats, nbands, krows, \
                      (int)nwg, SRCV, TT)
-#define AST_LAUNCH_P(STATS_, SRC_, SRCV)        \
+#define AST_LAUNCH_P(HAS_, STATS_, SRC_, SRCV)  \
   do {                                          \
-    if constexpr (kPulseKernels) {              \
+    if constexpr (HAS_) {                       \
       if (pulsed) {                             \
         AST_LAUNCH_K(STATS_, SRC_, true, SRCV); \
         break;                                  \
@@ -143,9 +145,9 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
           else AST_LAUNCH_K(true, kSrcF64, false, *src);
         }
       } else if (f32) {
-        AST_LAUNCH_P(false, kSrcF32, *src);
+        AST_LAUNCH_P(kPulseSrcKernels, false, kSrcF32, *src);
       } else {
-        AST_LAUNCH_P(false, kSrcF64, *src);
+        AST_LAUNCH_P(kPulseSrcKernels, false, kSrcF64, *src);
       }
     } else {
       return ast_fail(AST_ERR_ARG, "ast_step_f32: source planes with the learned start");
@@ -153,7 +155,7 @@ int launch_f32(const ast_grid_t& g, const Consts& c, const ExactCtx& ex, const f
   } else if (is_exact(METHOD) && stats != nullptr) {
     if constexpr (is_exact(METHOD)) AST_LAUNCH_K(true, kSrcNone, false, none);
   } else {
-    AST_LAUNCH_P(false, kSrcNone, none);
+    AST_LAUNCH_P(kPulseKernels, false, kSrcNone, none);
   }
 #undef AST_LAUNCH_P
 #undef AST_LAUNCH_K

@@ -134,20 +134,22 @@ def _wide_case(window):
     return g, s, u
 
 
+@pytest.mark.parametrize("method", [G.HLLC, G.EXACT])
 @pytest.mark.parametrize("tile", [8, 6, -5])
 @pytest.mark.parametrize("window", [True, False])
 @pytest.mark.parametrize("TT", [0.0013, 0.0031, 0.02])
-def test_pulsed_step_fp32_rows(dev, tile, window, TT):
+def test_pulsed_step_fp32_rows(dev, method, tile, window, TT):
     """A pulsed x side at time TT: rows below y_lim equal the plain FP32 step
     with the inflow density scaled by 1 + a sin(w TT) (1 from t_lim on), the
     other rows the unscaled one, bit for bit; 8-row strips take the fast
-    paths, 6-row ones the boundary path at the y edges."""
-    print(f"FP32 pulsed x side tile={tile} window={window} TT={TT}", flush=True)
+    paths, 6-row ones the boundary path at the y edges (the exact solver's:
+    everywhere but in whole groups of four rows)."""
+    print(f"FP32 pulsed x side method={method} tile={tile} window={window} TT={TT}", flush=True)
     g, s, u = _wide_case(window)
-    ps, pu, pT = _step32(g, s, u, 1e-3, TT, dev, tile)
+    ps, pu, pT = _step32(g, s, u, 1e-3, TT, dev, tile, method)
     g_lo, g_hi, rows = pulsed_equivalent(g, TT)
-    ls, lu, _ = _step32(g_lo, s, u, 1e-3, 0.0, dev, tile)
-    hs, hu, hT = _step32(g_hi, s, u, 1e-3, 0.0, dev, tile)
+    ls, lu, _ = _step32(g_lo, s, u, 1e-3, 0.0, dev, tile, method)
+    hs, hu, hT = _step32(g_hi, s, u, 1e-3, 0.0, dev, tile, method)
     assert 0 < rows.sum() < g.M
     assert np.array_equal(ps[rows], ls[rows]) and np.array_equal(pu[rows], lu[rows])
     assert np.array_equal(ps[~rows], hs[~rows]) and np.array_equal(pu[~rows], hu[~rows])
